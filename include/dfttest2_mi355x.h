@@ -1,0 +1,118 @@
+/*
+ * dfttest2_mi355x -- C ABI of the MI355X DFTTest block pipeline.
+ *
+ * A plan holds one set of DFTTest plugin parameters (the reference plugin's DFTTest
+ * arguments) on one GPU; process calls filter whole planes with the fused gfx950
+ * kernel.  All entry points are plain C: opaque handle, plain pointers and sizes,
+ * int status (0 = success, negative = error; dft2_last_error() names the error).
+ *
+ * Reference interfaces replaced (paths relative to the vs-dfttest2 reference tree):
+ *   dft2_plan_create    <- DFTTestCreate, hiprtc_source/source.cpp:710-934 (argument
+ *                          string hiprtc_source/source.cpp:1025-1046, defaults and
+ *                          checks :719-847)
+ *   dft2_plan_destroy   <- DFTTestFree, hiprtc_source/source.cpp:694-707
+ *   dft2_process_frame  <- DFTTestGetFrame, hiprtc_source/source.cpp:445-692 (one
+ *                          output frame of one plane from its T source frames)
+ *   dft2_process_batch  <- the same per-frame work over whole clips resident in device
+ *                          memory, many clips/planes per launch (no reference
+ *                          counterpart; the reference launches per frame and plane)
+ *   dft2_rdft           <- RDFT, cpu_source/source.cpp:269-340 (registered :390-398)
+ *   dft2_version        <- Version, cpu_source/source.cpp:342-354
+ */
+#ifndef DFTTEST2_MI355X_H
+#define DFTTEST2_MI355X_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DFT2_OK 0
+#define DFT2_ERR_ARG (-1)     /* invalid argument (message in dft2_last_error) */
+#define DFT2_ERR_HIP (-2)     /* HIP runtime error */
+#define DFT2_ERR_NOMEM (-3)   /* device or host allocation failed */
+#define DFT2_ERR_STATE (-4)   /* plan in an unusable state */
+
+/* Sample formats (VapourSynth sampleType / bitsPerSample). */
+#define DFT2_SAMPLE_INT 0
+#define DFT2_SAMPLE_FLOAT 1
+
+/* Precision of the DFT contractions. */
+#define DFT2_PRECISION_FP32 0      /* FP32 MFMA */
+#define DFT2_PRECISION_SPLIT_BF16 1 /* hi+lo BF16 operands, three products, FP32 accumulate */
+
+/* Plugin parameters, with the meaning and layout of the reference DFTTest arguments. */
+typedef struct dft2_params {
+    int radius;           /* "radius": 0..3, T = 2*radius+1 */
+    int block_size;       /* "block_size": must be 16 */
+    int block_step;       /* "block_step": 1..16 (sbsize - sosize) */
+    int filter_type;      /* "filter_type": 0..6 */
+    int zero_mean;        /* "zero_mean": 0 or 1 */
+    const double* window; /* "window": T*16*16 values, [t][y][x] */
+    int window_len;
+    const double* sigma;  /* "sigma": 1 value (scalar) or T*16*9 values, [t][ky][kx] */
+    int sigma_len;
+    double sigma2;        /* "sigma2" */
+    double pmin;          /* "pmin" */
+    double pmax;          /* "pmax" */
+    const double* window_freq; /* "window_freq": T*16*9*2 values [t][ky][kx][re,im]; needed if zero_mean */
+    int window_freq_len;
+} dft2_params;
+
+/* One plane of one clip resident in device memory.  Strides and pitches are in bytes.
+ * Output frames [frame_begin, frame_end) are written; source frames are taken from
+ * [0, n_frames) with the temporal clamp of the reference. */
+typedef struct dft2_job {
+    const void* src;
+    void* dst;
+    long long src_frame_pitch;
+    long long src_stride;
+    long long dst_frame_pitch;
+    long long dst_stride;
+    int width;
+    int height;
+    int n_frames;
+    int frame_begin;
+    int frame_end;
+} dft2_job;
+
+typedef struct dft2_plan dft2_plan;
+
+/* Creates a plan on `device_id`.  sample_type / bits_per_sample give the clip format
+ * (integer 8..16 bit, or float 32).  num_streams >= 1 concurrent calls are admitted,
+ * each with its own stream and scratch (the reference's num_streams). */
+int dft2_plan_create(const dft2_params* params, int sample_type, int bits_per_sample, int device_id,
+                     int num_streams, int precision, dft2_plan** out_plan);
+
+/* Releases the plan and its device memory. NULL is accepted. */
+int dft2_plan_destroy(dft2_plan* plan);
+
+/* Filters one output frame of one plane.  src_frames holds the T = 2*radius+1 source
+ * planes in temporal order (already clamped by the caller, as GetFrame gathers them).
+ * memory = 0: device pointers, work is queued on `stream` (hipStream_t; NULL = the
+ * plan's own stream, and the call waits for completion).  memory = 1: host pointers;
+ * the call stages through device memory and returns when dst is written. */
+int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, long long src_stride, void* dst,
+                       long long dst_stride, int width, int height, int memory, void* stream);
+
+/* Filters every job (device memory) with one fused launch per frame chunk, queued on
+ * `stream` (NULL = the plan's own stream, and the call waits for completion). */
+int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
+
+/* FP64 real DFT of `data` viewed as `shape` (ndim 1..3), unnormalised, exponent -2*pi*i,
+ * real along the last axis.  out receives prod(shape[:-1])*(shape[-1]/2+1) complex values
+ * as interleaved (re, im) doubles; out_len is its capacity in doubles. */
+int dft2_rdft(const double* data, int data_len, const int* shape, int ndim, double* out, int out_len);
+
+/* Version string of the library. */
+const char* dft2_version(void);
+
+/* Message of the last error on the calling thread ("" if none). */
+const char* dft2_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DFTTEST2_MI355X_H */

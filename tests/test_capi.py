@@ -1,0 +1,80 @@
+"""C ABI checks that need no GPU: the library loads, exports every declared entry point,
+validates arguments before touching a device, and its RDFT matches numpy."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from dfttest2_amd import _lib
+from dfttest2_amd.params import dfttest_params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "dfttest2_mi355x.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(dft2_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert set(names) == set(_lib.EXPORTED_SYMBOLS)
+    for name in names:
+        assert hasattr(lib, name), name
+
+
+def test_version_string():
+    assert "dfttest2_mi355x" in _lib.version()
+
+
+@pytest.mark.parametrize("shape", [(16, 16), (3, 16, 16), (7, 16, 16), (5,), (2, 3, 4)])
+def test_capi_rdft_matches_numpy(shape):
+    rng = np.random.default_rng(2)
+    data = rng.normal(size=shape)
+    spec = np.fft.rfftn(data)
+    out = _lib.rdft(data, shape).reshape(spec.shape + (2,))
+    np.testing.assert_allclose(out[..., 0], spec.real, atol=1e-9)
+    np.testing.assert_allclose(out[..., 1], spec.imag, atol=1e-9)
+
+
+def test_capi_rdft_rejects_bad_shape():
+    with pytest.raises(_lib.Dft2Error, match="reshape"):
+        _lib.rdft(np.zeros(10), (3, 4))
+
+
+def _params(**over):
+    p = dfttest_params()
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+@pytest.mark.parametrize(
+    "over,msg",
+    [
+        (dict(radius=4), "radius"),
+        (dict(block_size=8), "block_size"),
+        (dict(block_step=0), "block_step"),
+        (dict(filter_type=7), "filter_type"),
+        (dict(window=np.ones(100)), "window"),
+        (dict(sigma=np.ones(5)), "sigma"),
+        (dict(window_freq=None), "window_freq"),
+    ],
+)
+def test_plan_create_validates_arguments(over, msg):
+    with pytest.raises(_lib.Dft2Error, match=msg):
+        _lib.Plan(_params(**over), _lib.DFT2_SAMPLE_INT, 8)
+
+
+def test_plan_create_rejects_formats():
+    with pytest.raises(_lib.Dft2Error, match="integer"):
+        _lib.Plan(_params(), _lib.DFT2_SAMPLE_INT, 17)
+    with pytest.raises(_lib.Dft2Error, match="float"):
+        _lib.Plan(_params(), _lib.DFT2_SAMPLE_FLOAT, 16)
+    with pytest.raises(_lib.Dft2Error, match="num_streams"):
+        _lib.Plan(_params(), _lib.DFT2_SAMPLE_INT, 8, num_streams=0)

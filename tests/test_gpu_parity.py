@@ -1,0 +1,213 @@
+"""GPU parity of the fused gfx950 kernel, called through the C ABI.
+
+Bars: integer formats within 1 LSB, float32 within 1e-5 absolute (values in [0, 1]),
+against (a) the golden vectors made by the reference's own fused(), (b) the FP64
+oracle on the same seeded inputs, and (c) size-independent properties at full HD.
+"""
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from dfttest2_amd import _lib, synth
+from dfttest2_amd.dfttest2 import Backend, VideoClip, filter_clip, get_plan, plane_job
+from dfttest2_amd.params import dfttest_params
+from golden_util import fixture_names, load_fixture, max_err
+from oracle import dfttest_np as onp
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def announce(*what):
+    print("\n[case]", *what, flush=True)
+    sys.stdout.flush()
+
+
+def tol(is_float):
+    return 1e-5 if is_float else 1.0
+
+
+def gpu_filter(x, p, bits, is_float, be=None):
+    clip = VideoClip([x], "float" if is_float else "int", bits)
+    return filter_clip(clip, p, be).planes[0]
+
+
+def make_plane(fmt, shape, seed):
+    n, h, w = shape
+    if fmt == "u8":
+        return synth.int_plane(n, h, w, seed, 8, 6.0).numpy(), 8, False
+    if fmt == "u10":
+        return synth.int_plane(n, h, w, seed, 10, 12.0).numpy(), 10, False
+    if fmt == "u16":
+        return synth.int_plane(n, h, w, seed, 16, 700.0).numpy(), 16, False
+    return synth.float_plane(n, h, w, seed, 0.05).numpy(), 32, True
+
+
+@pytest.mark.parametrize("name", fixture_names())
+def test_golden_vectors(name):
+    announce("golden", name)
+    p, x, y, bits, is_float = load_fixture(name)
+    out = gpu_filter(x, p, bits, is_float)
+    assert out.dtype == y.dtype and out.shape == y.shape
+    assert max_err(out, y) <= tol(is_float)
+    assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= tol(is_float)
+
+
+CASES = [
+    ("u8", (6, 70, 90), dict()),
+    ("u8", (5, 64, 64), dict(tbsize=1)),
+    ("u10", (7, 66, 80), dict(ftype=1, tbsize=5, slocation=[0.0, 4, 0.5, 8, 1.0, 16])),
+    ("u16", (8, 60, 72), dict(tbsize=7, swin=4, twin=1)),
+    ("f32", (6, 75, 61), dict()),
+    ("f32", (5, 48, 100), dict(ftype=3, pmin=1.0, pmax=300.0)),
+    ("f32", (5, 52, 52), dict(ftype=4, pmin=2.0, pmax=400.0)),
+    ("u8", (5, 50, 70), dict(f0beta=0.5, sosize=8)),
+    ("u8", (5, 50, 70), dict(f0beta=0.8, sigma=16.0)),
+    ("u8", (5, 40, 50), dict(ftype=2, sigma=0.7, zmean=False)),
+    ("u8", (5, 47, 53), dict(sosize=14)),
+    ("u8", (5, 47, 53), dict(sosize=0)),
+    ("f32", (4, 40, 40), dict(sosize=4, swin=9)),
+    ("u8", (4, 44, 44), dict(ssystem=1, slocation=[0.0, 2.0, 1.0, 30.0])),
+]
+
+
+@pytest.mark.parametrize("fmt,shape,kw", CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(CASES)])
+def test_random_params_vs_oracle(fmt, shape, kw):
+    announce("oracle", fmt, shape, kw)
+    p = dfttest_params(**kw)
+    x, bits, is_float = make_plane(fmt, shape, seed=100 + len(kw))
+    out = gpu_filter(x, p, bits, is_float)
+    assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= tol(is_float)
+
+
+def test_asymmetric_sigma_array():
+    announce("asymmetric sigma")
+    p = dfttest_params()
+    rng = np.random.default_rng(7)
+    p.sigma = np.abs(rng.normal(1.0, 3.0, p.t_len * 16 * 9)) * 2.0
+    x, bits, is_float = make_plane("u8", (5, 50, 60), 31)
+    assert max_err(gpu_filter(x, p, bits, is_float), onp.process_plane(x, p, bits, is_float)) <= 1
+
+
+def test_yuv420_planes_and_passthrough():
+    announce("yuv420 planes")
+    planes = [t.numpy() for t in synth.yuv420_planes(4, 96, 64, seed=5)]
+    clip = VideoClip(planes, "int", 8)
+    p = dfttest_params(planes=[0, 2])
+    out = filter_clip(clip, p)
+    np.testing.assert_array_equal(out.planes[1], planes[1])
+    for i in (0, 2):
+        assert max_err(out.planes[i], onp.process_plane(planes[i], p, 8, False)) <= 1
+
+
+def test_frame_entry_host_and_device_match_batch():
+    announce("process_frame host/device")
+    p = dfttest_params(tbsize=5)
+    x, bits, _ = make_plane("u8", (6, 58, 66), 41)
+    full = gpu_filter(x, p, bits, False)
+    plan = get_plan(p, "int", 8, Backend.MI355X())
+    n, h, w = x.shape
+    for fr in (0, 2, 5):
+        idx = [min(max(fr - 2 + t, 0), n - 1) for t in range(5)]
+        srcs = [np.ascontiguousarray(x[i]) for i in idx]
+        out = np.zeros((h, w), np.uint8)
+        plan.process_frame([s.ctypes.data for s in srcs], w, out.ctypes.data, w, w, h, memory=1)
+        np.testing.assert_array_equal(out, full[fr])
+        dev = [torch.from_numpy(s).cuda() for s in srcs]
+        dout = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+        plan.process_frame([d.data_ptr() for d in dev], w, dout.data_ptr(), w, w, h, memory=0)
+        np.testing.assert_array_equal(dout.cpu().numpy(), full[fr])
+
+
+def test_batch_of_uneven_clips_matches_single_runs():
+    announce("batch uneven clips")
+    p = dfttest_params()
+    shapes = [(5, 40, 72), (9, 66, 48), (3, 30, 30), (7, 100, 36)]
+    xs = [synth.float_plane(*s, seed=60 + i).cuda() for i, s in enumerate(shapes)]
+    outs = [torch.empty_like(x) for x in xs]
+    plan = get_plan(p, "float", 32, Backend.MI355X())
+    plan.process_batch([plane_job(x, o) for x, o in zip(xs, outs)], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    for x, o in zip(xs, outs):
+        ref = onp.process_plane(x.cpu().numpy(), p, 32, True)
+        assert max_err(o.cpu().numpy(), ref) <= 1e-5
+
+
+def test_strided_rows_and_frame_range():
+    announce("strided rows, frame range")
+    p = dfttest_params()
+    x, bits, _ = make_plane("u8", (6, 45, 51), 71)
+    big = torch.zeros((6, 45, 64), dtype=torch.uint8, device="cuda")
+    big[:, :, :51] = torch.from_numpy(x).cuda()
+    src = big[:, :, :51]
+    dst_big = torch.full((6, 45, 64), 7, dtype=torch.uint8, device="cuda")
+    dst = dst_big[:, :, :51]
+    plan = get_plan(p, "int", 8, Backend.MI355X())
+    plan.process_batch([plane_job(src, dst, 2, 5)], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    ref = onp.process_plane(x, p, 8, False)
+    got = dst_big.cpu().numpy()
+    assert max_err(got[2:5, :, :51], ref[2:5]) <= 1
+    assert np.all(got[:2] == 7) and np.all(got[5:] == 7) and np.all(got[:, :, 51:] == 7)
+
+
+def test_short_clip_temporal_clamp():
+    announce("1-frame clip, tbsize 5")
+    p = dfttest_params(tbsize=5)
+    x, bits, _ = make_plane("u8", (1, 40, 40), 81)
+    assert max_err(gpu_filter(x, p, bits, False), onp.process_plane(x, p, bits, False)) <= 1
+
+
+def test_scratch_chunking_gives_same_result(monkeypatch):
+    announce("scratch chunking")
+    p = dfttest_params()
+    x = synth.int_plane(12, 90, 120, 91).cuda()
+    o1 = torch.empty_like(x)
+    o2 = torch.empty_like(x)
+    plan1 = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8)
+    monkeypatch.setenv("DFT2_SCRATCH_MB", "1")
+    plan2 = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8)
+    plan1.process_batch([plane_job(x, o1)])
+    plan2.process_batch([plane_job(x, o2)])
+    assert max_err(o1.cpu().numpy(), o2.cpu().numpy()) <= 1
+    plan1.close()
+    plan2.close()
+
+
+def test_too_small_plane_is_rejected():
+    announce("too small plane")
+    p = dfttest_params()
+    x = torch.zeros((2, 8, 8), dtype=torch.uint8, device="cuda")
+    plan = get_plan(p, "int", 8, Backend.MI355X())
+    with pytest.raises(_lib.Dft2Error, match="too small"):
+        plan.process_batch([plane_job(x, torch.empty_like(x))])
+
+
+@pytest.mark.parametrize("fmt", ["u8", "u10"])
+def test_full_hd_identity_is_exact(fmt):
+    # ftype 2 with sigma 1 is the identity (rectangular twin): integer output must equal
+    # the input sample for sample at the full 1080p luma size.
+    announce("1080p identity", fmt)
+    p = dfttest_params(ftype=2, sigma=1.0)
+    bits = 8 if fmt == "u8" else 10
+    x = synth.int_plane(3, 1080, 1920, 3, bits, 6.0, device="cuda")
+    out = torch.empty_like(x)
+    plan = get_plan(p, "int", bits, Backend.MI355X())
+    plan.process_batch([plane_job(x, out)], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.equal(out.to(torch.int32), x.to(torch.int32))
+
+
+def test_full_hd_constant_frames_pass_through():
+    announce("1080p constant")
+    p = dfttest_params(ftype=1, tbsize=5)
+    x = torch.full((5, 1080, 1920), 101, dtype=torch.uint8, device="cuda")
+    out = torch.empty_like(x)
+    plan = get_plan(p, "int", 8, Backend.MI355X())
+    plan.process_batch([plane_job(x, out)], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    assert torch.equal(out, x)

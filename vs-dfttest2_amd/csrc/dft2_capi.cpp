@@ -1,0 +1,794 @@
+// Host side of the dfttest2_mi355x C ABI (include/dfttest2_mi355x.h).
+//
+// Plan creation follows the reference DFTTestCreate (hiprtc_source/source.cpp:710-934):
+// argument checks, float conversion of window / sigma / window_freq, device upload.
+// Instead of compiling a kernel per radius at run time it selects precompiled gfx950
+// instantiations, and instead of the reference's host-side padding + per-frame launches
+// it describes whole planes (jobs) to the fused kernel.
+
+#include <algorithm>
+#include <cmath>
+#include <condition_variable>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dfttest2_mi355x.h"
+#include "dft2_internal.h"
+
+using namespace dft2;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                          \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if (e_ != hipSuccess) {                                                                                \
+            return fail(DFT2_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
+        }                                                                                                      \
+    } while (0)
+
+// cpu_source/getframe_impl.cpp:11-20
+int pad_size(int size, int bs, int step) {
+    return size + ((size % bs) ? bs - size % bs : 0) + std::max(bs - step, step) * 2;
+}
+
+int pad_num(int size, int bs, int step) {
+    return (pad_size(size, bs, step) - bs) / step + 1;
+}
+
+constexpr int kRing = 8;
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    float* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    JobDev* d_jobs = nullptr;
+    size_t d_jobs_cap = 0;
+    JobDev* h_jobs[kRing] = {};
+    size_t h_jobs_cap[kRing] = {};
+    hipEvent_t h_jobs_done[kRing] = {};
+    int ring_pos = 0;
+    void* stage = nullptr; // host-memory path: T input planes + output plane
+    size_t stage_bytes = 0;
+    bool busy = false;
+};
+
+} // namespace
+
+struct dft2_plan {
+    int device = 0;
+    int radius = 1;
+    int t_len = 3;
+    int block_step = 4;
+    int filter_type = 0;
+    int zero_mean = 0;
+    int asym = 0;
+    int sample_code = kU8;
+    int bits = 8;
+    int bytes_per_sample = 1;
+    float out_scale = 1.0f;
+    int peak = 255;
+    float sigma2 = 0, pmin = 0, pmax = 0, wf0 = 1.0f;
+    int tile_bw = 16, tile_bh = 16;
+    int region_w = 0, region_h = 0;
+    size_t lds_bytes = 0;
+    size_t scratch_cap = size_t(1) << 30;
+    float* d_consts = nullptr;
+    std::vector<Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv;
+};
+
+namespace {
+
+// Builds the constant table (layout in dft2_internal.h) in FP64 and rounds to float.
+void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, float scale) {
+    const double pi = 3.14159265358979323846;
+    const int T = p->t_len, r = p->radius;
+    c.assign(kConstFloats, 0.0f);
+    for (int x = 0; x < 16; x++) {
+        for (int n = 0; n < 16; n++) {
+            double v = n <= 8 ? std::cos(2 * pi * n * x / 16.0) : std::sin(2 * pi * (n - 8) * x / 16.0);
+            double w = (n == 0 || n == 8) ? 1.0 : 2.0;
+            c[kOffWX + x * 16 + n] = static_cast<float>(v);  // WX[x][n]
+            c[kOffWY + n * 16 + x] = static_cast<float>(v);  // WY[m][y]
+            c[kOffGX + n * 16 + x] = static_cast<float>(w * v); // GX[n][x]
+            c[kOffGY + x * 16 + n] = static_cast<float>(w * v); // GY[y][m]
+        }
+    }
+    for (int i = 0; i < T * 256; i++) {
+        // load_block computes (scale * window) * sample in float (cpu_source/getframe_impl.cpp:130-152)
+        c[kOffWIN + i] = scale * static_cast<float>(prm->window[i]);
+    }
+    for (int i = 0; i < 256; i++) {
+        c[kOffSYN + i] = static_cast<float>(prm->window[r * 256 + i]);
+    }
+    for (int j = 0; j < T; j++) {
+        for (int t = 0; t < T; t++) {
+            double v = j <= r ? std::cos(2 * pi * j * t / T) : std::sin(2 * pi * (j - r) * t / T);
+            c[kOffWT + j * T + t] = static_cast<float>(v);
+        }
+        double vr = j <= r ? std::cos(2 * pi * j * r / T) : std::sin(2 * pi * (j - r) * r / T);
+        c[kOffCT + j] = static_cast<float>((j == 0 ? 1.0 : 2.0) * vr);
+    }
+    auto sigma_at = [&](int kt, int ky, int kx) -> float {
+        if (prm->sigma_len == 1) {
+            return static_cast<float>(prm->sigma[0]);
+        }
+        return static_cast<float>(prm->sigma[(kt * 16 + ky) * 9 + kx]);
+    };
+    auto wf_at = [&](int kt, int ky, int kx, int part) -> float {
+        if (!p->zero_mean) {
+            return 0.0f;
+        }
+        return static_cast<float>(prm->window_freq[((kt * 16 + ky) * 9 + kx) * 2 + part]);
+    };
+    // Middle-stage ownership (see dft2_kernels.hip): lane = 8p + q, evaluations e = 0, 1,
+    // slots A (2e) and B (2e+1).
+    for (int g = 0; g <= r; g++) {
+        for (int lane = 0; lane < 64; lane++) {
+            const int pp = lane >> 3, q = lane & 7;
+            const bool ey = pp != 0, ex = q != 0;
+            for (int e = 0; e < 2; e++) {
+                const int ky = ey ? (e == 0 ? pp : (16 - pp) % 16) : (e == 0 ? 0 : 8);
+                int pos[2][3];
+                if (ex) {
+                    pos[0][0] = g, pos[0][1] = ky, pos[0][2] = q;
+                    pos[1][0] = (T - g) % T, pos[1][1] = (16 - ky) % 16, pos[1][2] = q;
+                } else {
+                    pos[0][0] = g, pos[0][1] = ky, pos[0][2] = 0;
+                    pos[1][0] = g, pos[1][1] = ky, pos[1][2] = 8;
+                }
+                for (int k = 0; k < 2; k++) {
+                    const int kt = pos[k][0], ky2 = pos[k][1], kx = pos[k][2];
+                    const bool edge = kx == 0 || kx == 8;
+                    float* e4 = &c[kOffTAB + (((g * 4 + 2 * e + k) * 64) + lane) * 4];
+                    e4[0] = sigma_at(kt, ky2, kx);
+                    e4[1] = edge ? sigma_at((T - kt) % T, (16 - ky2) % 16, kx) : e4[0];
+                    e4[2] = wf_at(kt, ky2, kx, 0);
+                    e4[3] = wf_at(kt, ky2, kx, 1);
+                }
+            }
+        }
+    }
+    // Asymmetric sigma: conjugate partners at kx = 0, 8 carry different sigmas.
+    p->asym = 0;
+    if (prm->sigma_len != 1) {
+        for (int kt = 0; kt < T && !p->asym; kt++) {
+            for (int ky = 0; ky < 16 && !p->asym; ky++) {
+                for (int kx : {0, 8}) {
+                    if (sigma_at(kt, ky, kx) != sigma_at((T - kt) % T, (16 - ky) % 16, kx)) {
+                        p->asym = 1;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+}
+
+Slot* acquire_slot(dft2_plan* p) {
+    std::unique_lock<std::mutex> lk(p->mu);
+    for (;;) {
+        for (auto& s : p->slots) {
+            if (!s.busy) {
+                s.busy = true;
+                return &s;
+            }
+        }
+        p->cv.wait(lk);
+    }
+}
+
+void release_slot(dft2_plan* p, Slot* s) {
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        s->busy = false;
+    }
+    p->cv.notify_one();
+}
+
+struct SlotGuard {
+    dft2_plan* p;
+    Slot* s;
+    ~SlotGuard() { release_slot(p, s); }
+};
+
+int ensure_device(void** ptr, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) {
+        return DFT2_OK;
+    }
+    if (*ptr) {
+        HIP_TRY(hipFree(*ptr));
+        *ptr = nullptr;
+        *cap = 0;
+    }
+    HIP_TRY(hipMalloc(ptr, bytes));
+    *cap = bytes;
+    return DFT2_OK;
+}
+
+int ensure_host(void** ptr, size_t* cap, size_t bytes) {
+    if (*cap >= bytes) {
+        return DFT2_OK;
+    }
+    if (*ptr) {
+        HIP_TRY(hipHostFree(*ptr));
+        *ptr = nullptr;
+        *cap = 0;
+    }
+    HIP_TRY(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    *cap = bytes;
+    return DFT2_OK;
+}
+
+int check_mirror(int size, int step, const char* what) {
+    int ps = pad_size(size, 16, step);
+    int o = (ps - size) / 2;
+    if (o > size - 1 || ps - o - size > size - 1) {
+        return fail(DFT2_ERR_ARG, "%s %d is too small for reflection padding of %d samples", what, size, o);
+    }
+    return DFT2_OK;
+}
+
+// Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
+int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream) {
+    const int n_jobs = static_cast<int>(jobs_in.size());
+    const long long region = static_cast<long long>(p->region_w) * p->region_h;
+    long long per_frame_items = 0;
+    int max_frames = 0;
+    for (const auto& j : jobs_in) {
+        per_frame_items += static_cast<long long>(j.tiles_x) * j.tiles_y;
+        max_frames = std::max(max_frames, j.f_count);
+    }
+    if (per_frame_items == 0 || max_frames == 0) {
+        return DFT2_OK;
+    }
+    long long frames_per_chunk = static_cast<long long>(p->scratch_cap / (per_frame_items * region * sizeof(float)));
+    frames_per_chunk = std::max(1LL, std::min<long long>(frames_per_chunk, max_frames));
+    const int n_chunks = static_cast<int>((max_frames + frames_per_chunk - 1) / frames_per_chunk);
+    const size_t scratch_bytes = static_cast<size_t>(frames_per_chunk * per_frame_items * region) * sizeof(float);
+
+    int rc = ensure_device(reinterpret_cast<void**>(&slot->scratch), &slot->scratch_bytes, scratch_bytes);
+    if (rc) {
+        return rc;
+    }
+    const size_t table_bytes = sizeof(JobDev) * static_cast<size_t>(n_jobs) * n_chunks;
+    // Device job table: reused across calls on this slot; stream order (or the wait on
+    // the slot's previous completion below) keeps earlier kernels from seeing new data.
+    if (slot->d_jobs_cap < table_bytes) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        rc = ensure_device(reinterpret_cast<void**>(&slot->d_jobs), &slot->d_jobs_cap, table_bytes);
+        if (rc) {
+            return rc;
+        }
+    }
+    const int ring = slot->ring_pos;
+    slot->ring_pos = (slot->ring_pos + 1) % kRing;
+    if (slot->h_jobs_done[ring]) {
+        HIP_TRY(hipEventSynchronize(slot->h_jobs_done[ring]));
+    } else {
+        HIP_TRY(hipEventCreateWithFlags(&slot->h_jobs_done[ring], hipEventDisableTiming));
+    }
+    rc = ensure_host(reinterpret_cast<void**>(&slot->h_jobs[ring]), &slot->h_jobs_cap[ring], table_bytes);
+    if (rc) {
+        return rc;
+    }
+    JobDev* h = slot->h_jobs[ring];
+    std::vector<long long> chunk_items(n_chunks), chunk_frames(n_chunks);
+    std::vector<int> chunk_maxpix(n_chunks);
+    for (int c = 0; c < n_chunks; c++) {
+        long long items = 0, jf = 0;
+        int maxpix = 0;
+        for (int k = 0; k < n_jobs; k++) {
+            JobDev j = jobs_in[k];
+            const int f0 = static_cast<int>(c * frames_per_chunk);
+            const int cnt = std::max(0, std::min<int>(static_cast<int>(frames_per_chunk), j.f_count - f0));
+            j.f_begin = j.f_begin + f0;
+            j.f_count = cnt;
+            j.item_begin = items;
+            j.frame_begin = jf;
+            items += static_cast<long long>(cnt) * j.tiles_x * j.tiles_y;
+            jf += cnt;
+            if (cnt > 0) {
+                maxpix = std::max(maxpix, j.width * j.height);
+            }
+            h[c * n_jobs + k] = j;
+        }
+        chunk_items[c] = items;
+        chunk_frames[c] = jf;
+        chunk_maxpix[c] = maxpix;
+    }
+    HIP_TRY(hipMemcpyAsync(slot->d_jobs, h, table_bytes, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipEventRecord(slot->h_jobs_done[ring], stream));
+
+    LaunchArgs a{};
+    a.consts = p->d_consts;
+    a.n_jobs = n_jobs;
+    a.scratch = slot->scratch;
+    a.block_step = p->block_step;
+    a.tile_bw = p->tile_bw;
+    a.tile_bh = p->tile_bh;
+    a.region_w = p->region_w;
+    a.region_h = p->region_h;
+    a.wf0 = p->wf0;
+    a.sigma2 = p->sigma2;
+    a.pmin = p->pmin;
+    a.pmax = p->pmax;
+    a.filter_type = p->filter_type;
+    a.zero_mean = p->zero_mean;
+    a.asym = p->asym;
+    a.out_scale = p->out_scale;
+    a.peak = p->peak;
+    for (int c = 0; c < n_chunks; c++) {
+        if (chunk_items[c] == 0) {
+            continue;
+        }
+        a.jobs = slot->d_jobs + static_cast<size_t>(c) * n_jobs;
+        HIP_TRY(launch_fused(a, p->radius, p->sample_code, chunk_items[c], p->lds_bytes, stream));
+        HIP_TRY(launch_finalize(a, p->sample_code, chunk_frames[c], chunk_maxpix[c], stream));
+    }
+    return DFT2_OK;
+}
+
+// Fills geometry of a JobDev from the user's job description.
+int make_job(const dft2_plan* p, const dft2_job& uj, JobDev& j) {
+    if (!uj.src || !uj.dst) {
+        return fail(DFT2_ERR_ARG, "job src/dst must not be NULL");
+    }
+    int rc = check_mirror(uj.width, p->block_step, "width");
+    if (rc) {
+        return rc;
+    }
+    rc = check_mirror(uj.height, p->block_step, "height");
+    if (rc) {
+        return rc;
+    }
+    if (uj.n_frames <= 0 || uj.frame_begin < 0 || uj.frame_end > uj.n_frames || uj.frame_begin > uj.frame_end) {
+        return fail(DFT2_ERR_ARG, "invalid frame range [%d, %d) of %d frames", uj.frame_begin, uj.frame_end,
+                    uj.n_frames);
+    }
+    const int bps = p->bytes_per_sample;
+    if (uj.src_stride % bps || uj.dst_stride % bps || uj.src_frame_pitch % bps || uj.dst_frame_pitch % bps) {
+        return fail(DFT2_ERR_ARG, "strides and pitches must be multiples of the sample size (%d)", bps);
+    }
+    if (uj.src_stride < static_cast<long long>(uj.width) * bps || uj.dst_stride < static_cast<long long>(uj.width) * bps) {
+        return fail(DFT2_ERR_ARG, "row stride smaller than the row");
+    }
+    std::memset(&j, 0, sizeof(j));
+    j.src = uj.src;
+    j.dst = uj.dst;
+    j.src_fp = uj.src_frame_pitch / bps;
+    j.src_rs = uj.src_stride / bps;
+    j.dst_fp = uj.dst_frame_pitch / bps;
+    j.dst_rs = uj.dst_stride / bps;
+    j.width = uj.width;
+    j.height = uj.height;
+    j.n_frames = uj.n_frames;
+    j.f_begin = uj.frame_begin;
+    j.f_count = uj.frame_end - uj.frame_begin;
+    j.pad_w = pad_size(uj.width, 16, p->block_step);
+    j.pad_h = pad_size(uj.height, 16, p->block_step);
+    j.nbx = pad_num(uj.width, 16, p->block_step);
+    j.nby = pad_num(uj.height, 16, p->block_step);
+    j.ox = (j.pad_w - uj.width) / 2;
+    j.oy = (j.pad_h - uj.height) / 2;
+    j.tiles_x = (j.nbx + p->tile_bw - 1) / p->tile_bw;
+    j.tiles_y = (j.nby + p->tile_bh - 1) / p->tile_bh;
+    j.use_fptr = 0;
+    return DFT2_OK;
+}
+
+int select_device(const dft2_plan* p) {
+    HIP_TRY(hipSetDevice(p->device));
+    return DFT2_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* dft2_last_error(void) {
+    return g_last_error.c_str();
+}
+
+const char* dft2_version(void) {
+    return "dfttest2_mi355x 0.1.0 (gfx950, fused FP32 MFMA)";
+}
+
+int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sample, int device_id, int num_streams,
+                     int precision, dft2_plan** out_plan) {
+    g_last_error.clear();
+    if (!prm || !out_plan) {
+        return fail(DFT2_ERR_ARG, "params and out_plan must not be NULL");
+    }
+    *out_plan = nullptr;
+    // Checks of DFTTestCreate (hiprtc_source/source.cpp:719-847).
+    if (prm->radius < 0 || prm->radius > 3) {
+        return fail(DFT2_ERR_ARG, "radius must be 0, 1, 2, or 3");
+    }
+    if (prm->block_size != 16) {
+        return fail(DFT2_ERR_ARG, "block_size must be 16");
+    }
+    if (prm->block_step < 1 || prm->block_step > 16) {
+        return fail(DFT2_ERR_ARG, "block_step must be in [1, 16]");
+    }
+    if (prm->filter_type < 0 || prm->filter_type > 6) {
+        return fail(DFT2_ERR_ARG, "filter_type must be in [0, 6]");
+    }
+    if (num_streams < 1) {
+        return fail(DFT2_ERR_ARG, "num_streams must be at least 1");
+    }
+    if (precision != DFT2_PRECISION_FP32) {
+        return fail(DFT2_ERR_ARG, "precision %d is not available in this build", precision);
+    }
+    const int T = 2 * prm->radius + 1;
+    if (!prm->window || prm->window_len != T * 256) {
+        return fail(DFT2_ERR_ARG, "window must have %d elements (got %d)", T * 256, prm->window_len);
+    }
+    if (!prm->sigma || (prm->sigma_len != 1 && prm->sigma_len != T * 16 * 9)) {
+        return fail(DFT2_ERR_ARG, "sigma must have 1 or %d elements (got %d)", T * 16 * 9, prm->sigma_len);
+    }
+    if (prm->zero_mean && (!prm->window_freq || prm->window_freq_len != T * 16 * 9 * 2)) {
+        return fail(DFT2_ERR_ARG, "window_freq with %d elements is required when zero_mean is enabled",
+                    T * 16 * 9 * 2);
+    }
+    if (sample_type == DFT2_SAMPLE_INT) {
+        if (bits_per_sample < 8 || bits_per_sample > 16) {
+            return fail(DFT2_ERR_ARG, "only 8-16 bit integer format input is supported");
+        }
+    } else if (sample_type == DFT2_SAMPLE_FLOAT) {
+        if (bits_per_sample != 32) {
+            return fail(DFT2_ERR_ARG, "only 32-bit float format input is supported");
+        }
+    } else {
+        return fail(DFT2_ERR_ARG, "invalid sample_type %d", sample_type);
+    }
+
+    auto* p = new dft2_plan();
+    p->device = device_id;
+    p->radius = prm->radius;
+    p->t_len = T;
+    p->block_step = prm->block_step;
+    p->filter_type = prm->filter_type;
+    p->zero_mean = prm->zero_mean ? 1 : 0;
+    p->bits = bits_per_sample;
+    if (sample_type == DFT2_SAMPLE_FLOAT) {
+        p->sample_code = kF32;
+        p->bytes_per_sample = 4;
+        p->out_scale = 255.0f;
+        p->peak = 0;
+    } else {
+        p->sample_code = bits_per_sample <= 8 ? kU8 : kU16;
+        p->bytes_per_sample = bits_per_sample <= 8 ? 1 : 2;
+        p->out_scale = 1.0f / static_cast<float>(1 << (bits_per_sample - 8));
+        p->peak = (1 << bits_per_sample) - 1;
+    }
+    p->sigma2 = static_cast<float>(prm->sigma2);
+    p->pmin = static_cast<float>(prm->pmin);
+    p->pmax = static_cast<float>(prm->pmax);
+    p->wf0 = p->zero_mean ? static_cast<float>(prm->window_freq[0]) : 1.0f;
+    if (p->zero_mean && p->wf0 == 0.0f) {
+        delete p;
+        return fail(DFT2_ERR_ARG, "window_freq[0] must be non-zero when zero_mean is enabled");
+    }
+    // Tile of blocks per workgroup: accumulator region at most ~100 samples per side.
+    const int tb = std::min(16, 84 / p->block_step + 1);
+    p->tile_bw = tb;
+    p->tile_bh = tb;
+    p->region_w = ((p->block_step * (tb - 1) + 16) + 3) / 4 * 4;
+    p->region_h = p->block_step * (tb - 1) + 16;
+    p->lds_bytes = fused_lds_bytes(p->radius, p->region_w, p->region_h);
+    if (const char* env = std::getenv("DFT2_SCRATCH_MB")) {
+        long mb = std::atol(env);
+        if (mb > 0) {
+            p->scratch_cap = static_cast<size_t>(mb) << 20;
+        }
+    }
+
+    std::vector<float> consts;
+    build_consts(p, prm, consts, p->out_scale);
+
+    int rc = select_device(p);
+    if (rc) {
+        delete p;
+        return rc;
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_consts), sizeof(float) * consts.size());
+    if (e == hipSuccess) {
+        e = hipMemcpy(p->d_consts, consts.data(), sizeof(float) * consts.size(), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        rc = fail(DFT2_ERR_HIP, "device setup failed: %s", hipGetErrorString(e));
+        dft2_plan_destroy(p);
+        return rc;
+    }
+    p->slots.resize(num_streams);
+    for (auto& s : p->slots) {
+        e = hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking);
+        if (e == hipSuccess) {
+            e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
+        }
+        if (e != hipSuccess) {
+            rc = fail(DFT2_ERR_HIP, "stream setup failed: %s", hipGetErrorString(e));
+            dft2_plan_destroy(p);
+            return rc;
+        }
+    }
+    *out_plan = p;
+    return DFT2_OK;
+}
+
+int dft2_plan_destroy(dft2_plan* p) {
+    if (!p) {
+        return DFT2_OK;
+    }
+    (void)hipSetDevice(p->device);
+    for (auto& s : p->slots) {
+        if (s.stream) {
+            (void)hipStreamSynchronize(s.stream);
+        }
+        for (int i = 0; i < kRing; i++) {
+            if (s.h_jobs_done[i]) {
+                (void)hipEventSynchronize(s.h_jobs_done[i]);
+                (void)hipEventDestroy(s.h_jobs_done[i]);
+            }
+            if (s.h_jobs[i]) {
+                (void)hipHostFree(s.h_jobs[i]);
+            }
+        }
+        if (s.done) {
+            (void)hipEventSynchronize(s.done);
+            (void)hipEventDestroy(s.done);
+        }
+        if (s.scratch) {
+            (void)hipFree(s.scratch);
+        }
+        if (s.d_jobs) {
+            (void)hipFree(s.d_jobs);
+        }
+        if (s.stage) {
+            (void)hipFree(s.stage);
+        }
+        if (s.stream) {
+            (void)hipStreamDestroy(s.stream);
+        }
+    }
+    if (p->d_consts) {
+        (void)hipFree(p->d_consts);
+    }
+    delete p;
+    return DFT2_OK;
+}
+
+int dft2_process_batch(dft2_plan* p, const dft2_job* jobs, int n_jobs, void* stream) {
+    g_last_error.clear();
+    if (!p) {
+        return fail(DFT2_ERR_ARG, "plan must not be NULL");
+    }
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
+        return fail(DFT2_ERR_ARG, "invalid job list");
+    }
+    std::vector<JobDev> jd(n_jobs);
+    for (int i = 0; i < n_jobs; i++) {
+        int rc = make_job(p, jobs[i], jd[i]);
+        if (rc) {
+            return rc;
+        }
+    }
+    int rc = select_device(p);
+    if (rc) {
+        return rc;
+    }
+    Slot* slot = acquire_slot(p);
+    SlotGuard guard{p, slot};
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : slot->stream;
+    if (stream) {
+        HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    }
+    rc = run_jobs(p, slot, jd, st);
+    if (rc) {
+        return rc;
+    }
+    HIP_TRY(hipEventRecord(slot->done, st));
+    if (!stream) {
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return DFT2_OK;
+}
+
+int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long src_stride, void* dst,
+                       long long dst_stride, int width, int height, int memory, void* stream) {
+    g_last_error.clear();
+    if (!p || !src_frames || !dst) {
+        return fail(DFT2_ERR_ARG, "plan, src_frames and dst must not be NULL");
+    }
+    const int T = p->t_len;
+    for (int t = 0; t < T; t++) {
+        if (!src_frames[t]) {
+            return fail(DFT2_ERR_ARG, "src_frames[%d] is NULL", t);
+        }
+    }
+    const int bps = p->bytes_per_sample;
+    dft2_job uj{};
+    uj.src = src_frames[0];
+    uj.dst = dst;
+    uj.width = width;
+    uj.height = height;
+    uj.n_frames = T;
+    uj.frame_begin = p->radius;
+    uj.frame_end = p->radius + 1;
+    uj.src_stride = src_stride;
+    uj.dst_stride = dst_stride;
+    JobDev j;
+    int rc = select_device(p);
+    if (rc) {
+        return rc;
+    }
+    if (memory == 0) {
+        rc = make_job(p, uj, j);
+        if (rc) {
+            return rc;
+        }
+        j.use_fptr = 1;
+        for (int t = 0; t < T; t++) {
+            j.fptr[t] = src_frames[t];
+        }
+        // dst is frame index r of a T-frame clip; shift the base so that r lands on dst.
+        j.dst_fp = 0;
+        Slot* slot = acquire_slot(p);
+        SlotGuard guard{p, slot};
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : slot->stream;
+        if (stream) {
+            HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+        }
+        rc = run_jobs(p, slot, {j}, st);
+        if (rc) {
+            return rc;
+        }
+        HIP_TRY(hipEventRecord(slot->done, st));
+        if (!stream) {
+            HIP_TRY(hipStreamSynchronize(st));
+        }
+        return DFT2_OK;
+    }
+    if (memory != 1) {
+        return fail(DFT2_ERR_ARG, "memory must be 0 (device) or 1 (host)");
+    }
+    // Host memory: stage T planes and the output through device memory (the reference
+    // copies T padded planes host->device per frame, hiprtc_source/source.cpp:570-573).
+    const long long row = static_cast<long long>(width) * bps;
+    const long long plane = row * height;
+    Slot* slot = acquire_slot(p);
+    SlotGuard guard{p, slot};
+    hipStream_t st = slot->stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    rc = ensure_device(&slot->stage, &slot->stage_bytes, static_cast<size_t>(plane) * (T + 1));
+    if (rc) {
+        return rc;
+    }
+    char* stage = static_cast<char*>(slot->stage);
+    for (int t = 0; t < T; t++) {
+        HIP_TRY(hipMemcpy2DAsync(stage + t * plane, row, src_frames[t], src_stride, row, height,
+                                 hipMemcpyHostToDevice, st));
+    }
+    uj.src = stage;
+    uj.src_stride = row;
+    uj.src_frame_pitch = plane;
+    uj.dst = stage + T * plane;
+    uj.dst_stride = row;
+    uj.dst_frame_pitch = 0;
+    rc = make_job(p, uj, j);
+    if (rc) {
+        return rc;
+    }
+    rc = run_jobs(p, slot, {j}, st);
+    if (rc) {
+        return rc;
+    }
+    HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, stage + T * plane, row, row, height, hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(slot->done, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return DFT2_OK;
+}
+
+int dft2_rdft(const double* data, int data_len, const int* shape, int ndim, double* out, int out_len) {
+    g_last_error.clear();
+    // Same transform as RDFT (cpu_source/source.cpp:269-340): naive separable DFT in FP64,
+    // real along the last axis, then complex along the others.
+    if (!data || !shape || !out || ndim < 1 || ndim > 3) {
+        return fail(DFT2_ERR_ARG, "\"shape\" must be an array of ints with 1, 2 or 3 values");
+    }
+    int dims[3] = {1, 1, 1};
+    long long size = 1;
+    for (int i = 0; i < ndim; i++) {
+        if (shape[i] <= 0) {
+            return fail(DFT2_ERR_ARG, "shape values must be positive");
+        }
+        dims[3 - ndim + i] = shape[i];
+        size *= shape[i];
+    }
+    if (size != data_len) {
+        return fail(DFT2_ERR_ARG, "cannot reshape array");
+    }
+    const int d0 = dims[0], d1 = dims[1], d2 = dims[2], h = d2 / 2 + 1;
+    const long long n_out = static_cast<long long>(d0) * d1 * h;
+    if (out_len < 2 * n_out) {
+        return fail(DFT2_ERR_ARG, "output needs %lld doubles", 2 * n_out);
+    }
+    const double pi = 3.14159265358979323846;
+    std::vector<double> re(n_out), im(n_out), tre(n_out), tim(n_out);
+    for (int a = 0; a < d0; a++) {
+        for (int b = 0; b < d1; b++) {
+            const double* src = data + (static_cast<long long>(a) * d1 + b) * d2;
+            for (int k = 0; k < h; k++) {
+                double sr = 0, si = 0;
+                for (int x = 0; x < d2; x++) {
+                    double ang = -2 * pi * static_cast<double>(k) * x / d2;
+                    sr += src[x] * std::cos(ang);
+                    si += src[x] * std::sin(ang);
+                }
+                re[(a * d1 + b) * h + k] = sr;
+                im[(a * d1 + b) * h + k] = si;
+            }
+        }
+    }
+    // axis 1 (length d1)
+    for (int a = 0; a < d0; a++) {
+        for (int k = 0; k < h; k++) {
+            for (int u = 0; u < d1; u++) {
+                double sr = 0, si = 0;
+                for (int b = 0; b < d1; b++) {
+                    double ang = -2 * pi * static_cast<double>(u) * b / d1;
+                    double c = std::cos(ang), s = std::sin(ang);
+                    double xr = re[(a * d1 + b) * h + k], xi = im[(a * d1 + b) * h + k];
+                    sr += xr * c - xi * s;
+                    si += xr * s + xi * c;
+                }
+                tre[(a * d1 + u) * h + k] = sr;
+                tim[(a * d1 + u) * h + k] = si;
+            }
+        }
+    }
+    // axis 0 (length d0)
+    for (int b = 0; b < d1; b++) {
+        for (int k = 0; k < h; k++) {
+            for (int u = 0; u < d0; u++) {
+                double sr = 0, si = 0;
+                for (int a = 0; a < d0; a++) {
+                    double ang = -2 * pi * static_cast<double>(u) * a / d0;
+                    double c = std::cos(ang), s = std::sin(ang);
+                    double xr = tre[(a * d1 + b) * h + k], xi = tim[(a * d1 + b) * h + k];
+                    sr += xr * c - xi * s;
+                    si += xr * s + xi * c;
+                }
+                re[(u * d1 + b) * h + k] = sr;
+                im[(u * d1 + b) * h + k] = si;
+            }
+        }
+    }
+    for (long long i = 0; i < n_out; i++) {
+        out[2 * i] = re[i];
+        out[2 * i + 1] = im[i];
+    }
+    return DFT2_OK;
+}
+
+} // extern "C"

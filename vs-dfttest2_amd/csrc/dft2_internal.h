@@ -1,0 +1,79 @@
+// Internal definitions shared by the host C ABI (dft2_capi.cpp) and the gfx950
+// kernels (dft2_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dft2 {
+
+// Workgroup shape of the fused kernel: 4 wave64 waves.
+constexpr int kWaves = 4;
+constexpr int kThreads = kWaves * 64;
+
+// Layout of the per-plan constant table (floats).  All offsets are multiples of 4.
+//   WX [x][n]   forward x transform, real basis n: 0..8 = cos(2 pi n x/16), 9..15 = sin(2 pi (n-8) x/16)
+//   WY [m][y]   forward y transform, same basis along y
+//   GX [n][x]   inverse x transform, WX^T with weights 1 (n = 0, 8) or 2 (others)
+//   GY [y][m]   inverse y transform, WY^T with the same weights
+//   WIN [t][y][x] analysis window times the sample scale (float32, as the plugin stores it)
+//   SYN [y][x]  synthesis window (window slice t = radius)
+//   WT [j][t]   temporal real basis: j = 0..r cos(2 pi j t/T), j = r+1..2r sin(2 pi (j-r) t/T)
+//   CT [j]      centre-slice inverse weights: WT[j][r] * (1 for j = 0, else 2)
+//   TAB [g][slot][lane][4]  per-lane gain table of the middle stage: sigma, partner sigma,
+//               Re and Im of window_freq at the coefficient the lane owns (see kernel).
+constexpr int kOffWX = 0;
+constexpr int kOffWY = 256;
+constexpr int kOffGX = 512;
+constexpr int kOffGY = 768;
+constexpr int kOffWIN = 1024;                 // up to 7*256
+constexpr int kOffSYN = kOffWIN + 7 * 256;    // 256
+constexpr int kOffWT = kOffSYN + 256;         // up to 49, padded to 64
+constexpr int kOffCT = kOffWT + 64;           // up to 7, padded to 16
+constexpr int kOffTAB = kOffCT + 16;          // up to 4*4*64*4
+constexpr int kTabFloats(int radius) { return (radius + 1) * 4 * 64 * 4; }
+constexpr int kConstFloats = kOffTAB + 4 * 4 * 64 * 4;
+
+// One plane of one clip in a launch (device copy of dft2_job plus derived geometry).
+struct JobDev {
+    const void* src;
+    void* dst;
+    long long src_fp, src_rs;   // frame pitch and row stride in samples
+    long long dst_fp, dst_rs;
+    int width, height, n_frames;
+    int f_begin, f_count;       // output frames of this chunk
+    int pad_w, pad_h, nbx, nby, ox, oy;
+    int tiles_x, tiles_y;
+    long long item_begin;       // first work item (fused kernel) of this job
+    long long frame_begin;      // first job-frame index (finalize kernel) of this job
+    int use_fptr;               // 1: source frame f is fptr[f] (f < 7) instead of src + f*src_fp
+    const void* fptr[7];
+};
+
+struct LaunchArgs {
+    const float* consts;
+    const JobDev* jobs;
+    int n_jobs;
+    float* scratch;             // per work item: region_h * region_w floats
+    int block_step;
+    int tile_bw, tile_bh;       // tile size in blocks
+    int region_w, region_h;     // accumulator region of a full tile
+    float wf0;                  // Re window_freq[0][0][0]
+    float sigma2, pmin, pmax;
+    int filter_type;
+    int zero_mean;
+    int asym;                   // sigma not symmetric under (kt,ky) -> (-kt,-ky) at kx = 0, 8
+    float out_scale;            // sample scale (2^(8-bits) for int, 255 for float)
+    int peak;                   // 2^bits - 1
+};
+
+// Sample type codes.
+enum SampleCode { kU8 = 0, kU16 = 1, kF32 = 2 };
+
+hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, long long n_items, size_t lds_bytes,
+                        hipStream_t stream);
+hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,
+                           hipStream_t stream);
+size_t fused_lds_bytes(int radius, int region_w, int region_h);
+
+} // namespace dft2

@@ -1,0 +1,184 @@
+"""ctypes binding of the dfttest2_mi355x C ABI (include/dfttest2_mi355x.h).
+
+The native library is built in-tree by vs-dfttest2_amd/build_native.py.  There is no
+fallback: if the library is missing, loading raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_native", "libdfttest2_mi355x.so")
+
+DFT2_SAMPLE_INT = 0
+DFT2_SAMPLE_FLOAT = 1
+DFT2_PRECISION_FP32 = 0
+DFT2_PRECISION_SPLIT_BF16 = 1
+
+EXPORTED_SYMBOLS = (
+    "dft2_plan_create",
+    "dft2_plan_destroy",
+    "dft2_process_frame",
+    "dft2_process_batch",
+    "dft2_rdft",
+    "dft2_version",
+    "dft2_last_error",
+)
+
+_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+class Dft2Params(ctypes.Structure):
+    _fields_ = [
+        ("radius", ctypes.c_int),
+        ("block_size", ctypes.c_int),
+        ("block_step", ctypes.c_int),
+        ("filter_type", ctypes.c_int),
+        ("zero_mean", ctypes.c_int),
+        ("window", _dbl_p),
+        ("window_len", ctypes.c_int),
+        ("sigma", _dbl_p),
+        ("sigma_len", ctypes.c_int),
+        ("sigma2", ctypes.c_double),
+        ("pmin", ctypes.c_double),
+        ("pmax", ctypes.c_double),
+        ("window_freq", _dbl_p),
+        ("window_freq_len", ctypes.c_int),
+    ]
+
+
+class Dft2Job(ctypes.Structure):
+    _fields_ = [
+        ("src", ctypes.c_void_p),
+        ("dst", ctypes.c_void_p),
+        ("src_frame_pitch", ctypes.c_longlong),
+        ("src_stride", ctypes.c_longlong),
+        ("dst_frame_pitch", ctypes.c_longlong),
+        ("dst_stride", ctypes.c_longlong),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("n_frames", ctypes.c_int),
+        ("frame_begin", ctypes.c_int),
+        ("frame_end", ctypes.c_int),
+    ]
+
+
+class Dft2Error(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads the native library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise Dft2Error(f"native library not built: {LIB_PATH} (run vs-dfttest2_amd/build_native.py)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.dft2_plan_create.argtypes = [
+        ctypes.POINTER(Dft2Params), ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.POINTER(ctypes.c_void_p),
+    ]
+    lib.dft2_plan_create.restype = ctypes.c_int
+    lib.dft2_plan_destroy.argtypes = [ctypes.c_void_p]
+    lib.dft2_plan_destroy.restype = ctypes.c_int
+    lib.dft2_process_frame.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_longlong, ctypes.c_void_p, ctypes.c_longlong,
+        ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+    ]
+    lib.dft2_process_frame.restype = ctypes.c_int
+    lib.dft2_process_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p]
+    lib.dft2_process_batch.restype = ctypes.c_int
+    lib.dft2_rdft.argtypes = [_dbl_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _dbl_p, ctypes.c_int]
+    lib.dft2_rdft.restype = ctypes.c_int
+    lib.dft2_version.argtypes = []
+    lib.dft2_version.restype = ctypes.c_char_p
+    lib.dft2_last_error.argtypes = []
+    lib.dft2_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = load().dft2_last_error().decode(errors="replace")
+        raise Dft2Error(f"dfttest2_mi355x error {rc}: {msg}")
+
+
+def version() -> str:
+    return load().dft2_version().decode()
+
+
+def rdft(data: Sequence[float], shape: Sequence[int]) -> np.ndarray:
+    """RDFT through the C ABI (same contract as the plugin's RDFT)."""
+    arr = np.ascontiguousarray(np.asarray(data, dtype=np.float64).reshape(-1))
+    shp = (ctypes.c_int * len(shape))(*[int(s) for s in shape])
+    n_out = int(np.prod(shape[:-1])) * (int(shape[-1]) // 2 + 1) * 2 if len(shape) else 0
+    out = np.zeros(max(n_out, 1), dtype=np.float64)
+    _check(load().dft2_rdft(arr.ctypes.data_as(_dbl_p), arr.size, shp, len(shape), out.ctypes.data_as(_dbl_p),
+                            out.size))
+    return out[:n_out]
+
+
+class Plan:
+    """One set of DFTTest plugin parameters on one GPU (dft2_plan_create)."""
+
+    def __init__(self, params, sample_type: int, bits_per_sample: int, device_id: int = 0, num_streams: int = 1,
+                 precision: int = DFT2_PRECISION_FP32):
+        lib = load()
+        self._keep = []
+        window = np.ascontiguousarray(params.window, dtype=np.float64)
+        sigma = np.ascontiguousarray(np.atleast_1d(params.sigma), dtype=np.float64)
+        wf = params.window_freq
+        wf = None if wf is None else np.ascontiguousarray(wf, dtype=np.float64)
+        self._keep += [window, sigma, wf]
+        prm = Dft2Params(
+            radius=params.radius,
+            block_size=params.block_size,
+            block_step=params.block_step,
+            filter_type=params.filter_type,
+            zero_mean=int(bool(params.zero_mean)),
+            window=window.ctypes.data_as(_dbl_p),
+            window_len=window.size,
+            sigma=sigma.ctypes.data_as(_dbl_p),
+            sigma_len=sigma.size,
+            sigma2=params.sigma2,
+            pmin=params.pmin,
+            pmax=params.pmax,
+            window_freq=wf.ctypes.data_as(_dbl_p) if wf is not None else _dbl_p(),
+            window_freq_len=0 if wf is None else wf.size,
+        )
+        handle = ctypes.c_void_p()
+        _check(lib.dft2_plan_create(ctypes.byref(prm), sample_type, bits_per_sample, device_id, num_streams,
+                                    precision, ctypes.byref(handle)))
+        self._h = handle
+        self.radius = params.radius
+        self.device_id = device_id
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dft2_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def process_batch(self, jobs: List[Dft2Job], stream: Optional[int] = None):
+        arr = (Dft2Job * len(jobs))(*jobs)
+        _check(load().dft2_process_batch(self._h, arr, len(jobs), ctypes.c_void_p(stream or 0)))
+
+    def process_frame(self, src_ptrs: Sequence[int], src_stride: int, dst: int, dst_stride: int, width: int,
+                      height: int, memory: int, stream: Optional[int] = None):
+        ptrs = (ctypes.c_void_p * len(src_ptrs))(*src_ptrs)
+        _check(load().dft2_process_frame(self._h, ptrs, src_stride, ctypes.c_void_p(dst), dst_stride, width, height,
+                                         memory, ctypes.c_void_p(stream or 0)))
