@@ -78,11 +78,26 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision", choices=["fp32", "split_bf16"], default="fp32")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path has no multi-GPU form here (use --gpus 1)",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_probe_mb:
+        import torch
+
+        n = args.copy_probe_mb << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n, "dispatches": 3}), flush=True)
+        return
 
     # Whole clip in one fused launch: scratch for 240 frames of tile regions.
     os.environ.setdefault("DFT2_SCRATCH_MB", "8192")
@@ -98,7 +113,7 @@ def main():
     print(f"[bench] generating {FRAMES} frames of {WIDTH}x{HEIGHT} YUV420 u8 on {dev}", file=sys.stderr, flush=True)
     planes = synth.yuv420_planes(FRAMES, WIDTH, HEIGHT, seed=1, bits=8, sd_lsb=6.0, device=dev)
     outs = [torch.empty_like(p) for p in planes]
-    plan = get_plan(params, "int", 8, Backend.MI355X())
+    plan = get_plan(params, "int", 8, Backend.MI355X(precision=args.precision))
     jobs = [plane_job(p, o) for p, o in zip(planes, outs)]
     stream = torch.cuda.current_stream(dev)
     handle = stream.cuda_stream
@@ -144,7 +159,7 @@ def main():
                         "ftype 0 sigma 8 sbsize 16 sosize 12 tbsize 3 swin 0 twin 7 zmean",
             "frames": FRAMES,
             "fps": round(FRAMES / (ms_per_step / 1e3), 1),
-            "precision": "fp32",
+            "precision": args.precision,
         },
         "roofline": {
             "bound": "mfma",

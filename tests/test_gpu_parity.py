@@ -211,3 +211,32 @@ def test_full_hd_constant_frames_pass_through():
     plan.process_batch([plane_job(x, out)], torch.cuda.current_stream().cuda_stream)
     torch.cuda.synchronize()
     assert torch.equal(out, x)
+
+
+SPLIT_CASES = [
+    ("u8", (5, 64, 80), dict()),
+    ("u10", (6, 60, 72), dict(ftype=1, tbsize=5, slocation=[0.0, 4, 0.5, 8, 1.0, 16])),
+    ("u8", (8, 50, 66), dict(tbsize=7, swin=1)),
+    ("u16", (4, 48, 48), dict(tbsize=1, ftype=3, pmax=1e4)),
+    ("f32", (5, 55, 64), dict()),
+]
+
+
+@pytest.mark.parametrize("fmt,shape,kw", SPLIT_CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(SPLIT_CASES)])
+def test_split_bf16_vs_oracle(fmt, shape, kw):
+    # Split BF16 (hi + lo operands, three products, FP32 accumulation) keeps about 16
+    # significant bits per operand: integer formats stay within 1 LSB; float32 output is
+    # held to 5e-5 absolute (the FP32 form meets 1e-5).
+    announce("split-bf16", fmt, shape, kw)
+    p = dfttest_params(**kw)
+    x, bits, is_float = make_plane(fmt, shape, seed=200 + len(kw))
+    out = gpu_filter(x, p, bits, is_float, Backend.MI355X(precision="split_bf16"))
+    bound = 5e-5 if is_float else 1.0
+    assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= bound
+
+
+def test_split_bf16_golden_defaults():
+    announce("split-bf16 golden u8_defaults")
+    p, x, y, bits, is_float = load_fixture("u8_defaults")
+    out = gpu_filter(x, p, bits, is_float, Backend.MI355X(precision="split_bf16"))
+    assert max_err(out, y) <= 1

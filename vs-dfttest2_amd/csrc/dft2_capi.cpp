@@ -81,6 +81,7 @@ struct dft2_plan {
     int zero_mean = 0;
     int asym = 0;
     int sample_code = kU8;
+    int precision = DFT2_PRECISION_FP32;
     int bits = 8;
     int bytes_per_sample = 1;
     float out_scale = 1.0f;
@@ -342,7 +343,7 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
             continue;
         }
         a.jobs = slot->d_jobs + static_cast<size_t>(c) * n_jobs;
-        HIP_TRY(launch_fused(a, p->radius, p->sample_code, chunk_items[c], p->lds_bytes, stream));
+        HIP_TRY(launch_fused(a, p->radius, p->sample_code, p->precision, chunk_items[c], p->lds_bytes, stream));
         HIP_TRY(launch_finalize(a, p->sample_code, chunk_frames[c], chunk_maxpix[c], stream));
     }
     return DFT2_OK;
@@ -410,7 +411,7 @@ const char* dft2_last_error(void) {
 }
 
 const char* dft2_version(void) {
-    return "dfttest2_mi355x 0.1.0 (gfx950, fused FP32 MFMA)";
+    return "dfttest2_mi355x 0.1.0 (gfx950, fused FP32 / split-BF16 MFMA)";
 }
 
 int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sample, int device_id, int num_streams,
@@ -436,8 +437,8 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
     if (num_streams < 1) {
         return fail(DFT2_ERR_ARG, "num_streams must be at least 1");
     }
-    if (precision != DFT2_PRECISION_FP32) {
-        return fail(DFT2_ERR_ARG, "precision %d is not available in this build", precision);
+    if (precision != DFT2_PRECISION_FP32 && precision != DFT2_PRECISION_SPLIT_BF16) {
+        return fail(DFT2_ERR_ARG, "precision must be 0 (FP32) or 1 (split BF16)");
     }
     const int T = 2 * prm->radius + 1;
     if (!prm->window || prm->window_len != T * 256) {
@@ -470,6 +471,7 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
     p->filter_type = prm->filter_type;
     p->zero_mean = prm->zero_mean ? 1 : 0;
     p->bits = bits_per_sample;
+    p->precision = precision;
     if (sample_type == DFT2_SAMPLE_FLOAT) {
         p->sample_code = kF32;
         p->bytes_per_sample = 4;

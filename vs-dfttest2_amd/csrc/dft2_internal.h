@@ -70,8 +70,8 @@ struct LaunchArgs {
 // Sample type codes.
 enum SampleCode { kU8 = 0, kU16 = 1, kF32 = 2 };
 
-hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, long long n_items, size_t lds_bytes,
-                        hipStream_t stream);
+hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, int precision, long long n_items,
+                        size_t lds_bytes, hipStream_t stream);
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,
                            hipStream_t stream);
 size_t fused_lds_bytes(int radius, int region_w, int region_h);

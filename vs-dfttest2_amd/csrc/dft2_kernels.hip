@@ -33,9 +33,63 @@ namespace dft2 {
 namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short shortx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// One lane's share of a 16-long contraction operand: the value at K slot (lane/16, s)
+// for s = 0..3, plus its split into BF16 hi + lo parts for the split-precision form.
+struct Opnd {
+    float f[4];
+    bf16x4 hi, lo;
+};
+
+template <int PREC>
+__device__ __forceinline__ Opnd make_opnd(float a0, float a1, float a2, float a3) {
+    Opnd o;
+    o.f[0] = a0;
+    o.f[1] = a1;
+    o.f[2] = a2;
+    o.f[3] = a3;
+    if constexpr (PREC == 1) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            const __bf16 h = static_cast<__bf16>(o.f[s]);
+            o.hi[s] = h;
+            o.lo[s] = static_cast<__bf16>(o.f[s] - static_cast<float>(h));
+        }
+    }
+    return o;
+}
+
+// acc += A * B over one 16-long contraction (4 K slots per lane x 4 lane groups).
+// PREC 0: four FP32 MFMA 16x16x4.  PREC 1: split BF16 with FP32 accumulation,
+// A*B ~ Ahi*Bhi + Alo*Bhi + Ahi*Blo, three BF16 MFMA 16x16x16.  The K slot pairing
+// (lane group, s) is the same in every form, so operands need no reshuffle.
+// The K-packed alternative (one 16x16x32 for [Ahi | Alo] x [Bhi | Bhi] plus one 16x16x16)
+// returned corrupted products on gfx950 when the compiler scheduled a VALU write to a
+// source register of the 16x16x32 right after it; with wait states inserted it was
+// correct, and three 16x16x16 ran at the same speed, so this form is used.
+template <int PREC>
+__device__ __forceinline__ floatx4 mm16(const Opnd& A, const Opnd& B, floatx4 acc) {
+    if constexpr (PREC == 0) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            acc = mfma4(A.f[s], B.f[s], acc);
+        }
+    } else {
+        const shortx4 ahi = __builtin_bit_cast(shortx4, A.hi);
+        const shortx4 alo = __builtin_bit_cast(shortx4, A.lo);
+        const shortx4 bhi = __builtin_bit_cast(shortx4, B.hi);
+        const shortx4 blo = __builtin_bit_cast(shortx4, B.lo);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
+    }
+    return acc;
 }
 
 // Completes this wave's outstanding LDS operations and stops the compiler from moving
@@ -101,7 +155,7 @@ __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, floa
     im = xi * g + gf * e.w;
 }
 
-template <int R, typename Tin, int FTM>
+template <int R, typename Tin, int FTM, int PREC>
 __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
     constexpr int kWbuf = (T * 256 > 320) ? T * 256 : 320;
@@ -163,6 +217,10 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
             win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
         }
     }
+    const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
+    const Opnd WYo = make_opnd<PREC>(wy[0], wy[1], wy[2], wy[3]);
+    const Opnd GXo = make_opnd<PREC>(gx[0], gx[1], gx[2], gx[3]);
+    const Opnd GYo = make_opnd<PREC>(gy[0], gy[1], gy[2], gy[3]);
     float wt[T][T];
     float ct[T];
 #pragma unroll
@@ -212,17 +270,11 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const Tin* row = frame[t] + row_off;
-            floatx4 d1 = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                d1 = mfma4(win[t][s] * load_sample(row + col[s]), wx[s], d1);
-            }
-            floatx4 d = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-            for (int s = 0; s < 4; s++) {
-                d = mfma4(wy[s], d1[s], d);
-            }
-            d2[t] = d;
+            const Opnd in = make_opnd<PREC>(win[t][0] * load_sample(row + col[0]), win[t][1] * load_sample(row + col[1]),
+                                            win[t][2] * load_sample(row + col[2]), win[t][3] * load_sample(row + col[3]));
+            const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            const floatx4 d1 = mm16<PREC>(in, WXo, zero);
+            d2[t] = mm16<PREC>(WYo, make_opnd<PREC>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
 
         // ---- temporal real basis, staged as [j][p][q][b*2+c] ----
@@ -326,16 +378,9 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
 
         // ---- inverse: out = GY * (V * GX) ----
         const float4 v = *reinterpret_cast<const float4*>(&wbuf[l16 * 20 + 4 * lg]);
-        floatx4 f1 = {0.0f, 0.0f, 0.0f, 0.0f};
-        f1 = mfma4(v.x, gx[0], f1);
-        f1 = mfma4(v.y, gx[1], f1);
-        f1 = mfma4(v.z, gx[2], f1);
-        f1 = mfma4(v.w, gx[3], f1);
-        floatx4 o = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            o = mfma4(gy[s], f1[s], o);
-        }
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const floatx4 f1 = mm16<PREC>(make_opnd<PREC>(v.x, v.y, v.z, v.w), GXo, zero);
+        const floatx4 o = mm16<PREC>(GYo, make_opnd<PREC>(f1[0], f1[1], f1[2], f1[3]), zero);
 
         // ---- synthesis window + overlap-add into the tile accumulator ----
         float* arow = acc + (bly * S + 4 * lg) * rw + blx * S + l16;
@@ -418,9 +463,9 @@ __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
     store_sample(dst, sum, a);
 }
 
-template <int R, typename Tin, int FTM>
+template <int R, typename Tin, int FTM, int PREC>
 hipError_t launch_one(const LaunchArgs& a, long long n_items, size_t lds, hipStream_t stream) {
-    auto kern = fused_kernel<R, Tin, FTM>;
+    auto kern = fused_kernel<R, Tin, FTM, PREC>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
@@ -432,26 +477,35 @@ hipError_t launch_one(const LaunchArgs& a, long long n_items, size_t lds, hipStr
     return hipGetLastError();
 }
 
-template <int R, typename Tin>
+template <int R, typename Tin, int PREC>
 hipError_t launch_ft(const LaunchArgs& a, long long n_items, size_t lds, hipStream_t stream) {
     if (a.filter_type == 0) {
-        return launch_one<R, Tin, 0>(a, n_items, lds, stream);
+        return launch_one<R, Tin, 0, PREC>(a, n_items, lds, stream);
     }
     if (a.filter_type == 1) {
-        return launch_one<R, Tin, 1>(a, n_items, lds, stream);
+        return launch_one<R, Tin, 1, PREC>(a, n_items, lds, stream);
     }
-    return launch_one<R, Tin, 2>(a, n_items, lds, stream);
+    return launch_one<R, Tin, 2, PREC>(a, n_items, lds, stream);
+}
+
+template <int R, typename Tin>
+hipError_t launch_prec(const LaunchArgs& a, int precision, long long n_items, size_t lds, hipStream_t stream) {
+    if (precision == 1) {
+        return launch_ft<R, Tin, 1>(a, n_items, lds, stream);
+    }
+    return launch_ft<R, Tin, 0>(a, n_items, lds, stream);
 }
 
 template <int R>
-hipError_t launch_type(const LaunchArgs& a, int sample_code, long long n_items, size_t lds, hipStream_t stream) {
+hipError_t launch_type(const LaunchArgs& a, int sample_code, int precision, long long n_items, size_t lds,
+                       hipStream_t stream) {
     switch (sample_code) {
     case kU8:
-        return launch_ft<R, uint8_t>(a, n_items, lds, stream);
+        return launch_prec<R, uint8_t>(a, precision, n_items, lds, stream);
     case kU16:
-        return launch_ft<R, uint16_t>(a, n_items, lds, stream);
+        return launch_prec<R, uint16_t>(a, precision, n_items, lds, stream);
     default:
-        return launch_ft<R, float>(a, n_items, lds, stream);
+        return launch_prec<R, float>(a, precision, n_items, lds, stream);
     }
 }
 
@@ -463,17 +517,17 @@ size_t fused_lds_bytes(int radius, int region_w, int region_h) {
     return sizeof(float) * (static_cast<size_t>(region_w) * region_h + (radius + 1) * 4 * 64 * 4 + kWaves * wbuf);
 }
 
-hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, long long n_items, size_t lds_bytes,
-                        hipStream_t stream) {
+hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, int precision, long long n_items,
+                        size_t lds_bytes, hipStream_t stream) {
     switch (radius) {
     case 0:
-        return launch_type<0>(a, sample_code, n_items, lds_bytes, stream);
+        return launch_type<0>(a, sample_code, precision, n_items, lds_bytes, stream);
     case 1:
-        return launch_type<1>(a, sample_code, n_items, lds_bytes, stream);
+        return launch_type<1>(a, sample_code, precision, n_items, lds_bytes, stream);
     case 2:
-        return launch_type<2>(a, sample_code, n_items, lds_bytes, stream);
+        return launch_type<2>(a, sample_code, precision, n_items, lds_bytes, stream);
     default:
-        return launch_type<3>(a, sample_code, n_items, lds_bytes, stream);
+        return launch_type<3>(a, sample_code, precision, n_items, lds_bytes, stream);
     }
 }
 
