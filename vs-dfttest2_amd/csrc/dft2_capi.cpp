@@ -329,7 +329,7 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
     a.tile_bh = p->tile_bh;
     a.region_w = p->region_w;
     a.region_h = p->region_h;
-    a.wf0 = p->wf0;
+    a.inv_wf0 = 1.0f / p->wf0;
     a.sigma2 = p->sigma2;
     a.pmin = p->pmin;
     a.pmax = p->pmax;

@@ -7,8 +7,8 @@
 
 namespace dft2 {
 
-// Workgroup shape of the fused kernel: 4 wave64 waves.
-constexpr int kWaves = 4;
+// Workgroup shape of the fused kernel: 8 wave64 waves sharing one tile accumulator.
+constexpr int kWaves = 8;
 constexpr int kThreads = kWaves * 64;
 
 // Layout of the per-plan constant table (floats).  All offsets are multiples of 4.
@@ -58,7 +58,7 @@ struct LaunchArgs {
     int block_step;
     int tile_bw, tile_bh;       // tile size in blocks
     int region_w, region_h;     // accumulator region of a full tile
-    float wf0;                  // Re window_freq[0][0][0]
+    float inv_wf0;              // 1 / Re window_freq[0][0][0]
     float sigma2, pmin, pmax;
     int filter_type;
     int zero_mean;

@@ -104,9 +104,14 @@ __device__ __forceinline__ int mirror(int v, int n) {
     return v >= n ? 2 * (n - 1) - v : v;
 }
 
+// Source planes live in device global memory; loading through an address-space-1
+// pointer gives global (not flat) loads.
 template <typename Tin>
-__device__ __forceinline__ float load_sample(const Tin* p) {
-    return static_cast<float>(*p);
+using gptr_t = const __attribute__((address_space(1))) Tin*;
+
+template <typename Tin>
+__device__ __forceinline__ gptr_t<Tin> to_global(const Tin* p) {
+    return (gptr_t<Tin>)(p);
 }
 
 // Gain of one coefficient (cpu_source/kernel.hpp:1826-1890).  FTM 0 and 1 are the
@@ -233,13 +238,33 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     }
 
     // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
-    const Tin* frame[T];
+    gptr_t<Tin> frame[T];
 #pragma unroll
     for (int t = 0; t < T; t++) {
         int fi = min(max(n - R + t, 0), job.n_frames - 1);
-        frame[t] = job.use_fptr ? reinterpret_cast<const Tin*>(job.fptr[fi])
-                                : reinterpret_cast<const Tin*>(job.src) + static_cast<long long>(fi) * job.src_fp;
+        frame[t] = to_global(job.use_fptr ? reinterpret_cast<const Tin*>(job.fptr[fi])
+                                          : reinterpret_cast<const Tin*>(job.src) + static_cast<long long>(fi) * job.src_fp);
     }
+    // Samples of one block for this lane: row y = lane%16, columns 4*(lane/16)..+3 of
+    // every slice, with the mirror padding applied in the address.
+    auto fetch = [&](int blk, float (&dst)[T][4]) {
+        const int fy = blk / nbx;
+        const int fx = blk - fy * nbx;
+        const long long roff =
+            static_cast<long long>(mirror((by0 + fy) * S + l16 - job.oy, job.height)) * job.src_rs;
+        int cols[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            cols[s] = mirror((bx0 + fx) * S + 4 * lg + s - job.ox, job.width);
+        }
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                dst[t][s] = static_cast<float>(frame[t][roff + cols[s]]);
+            }
+        }
+    };
 
     // Middle-stage ownership: lane = 8*p + q owns y pair p (rows m = p, p+8 of the real
     // basis) and x pair q (columns n = q, q+8).  Pair 0 is (cos 0, cos 8) on both axes,
@@ -251,27 +276,36 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     const int wq = l16 & 7;
     const int wc = l16 >> 3;
 
+    // The next block's samples are fetched one iteration ahead, so their load latency
+    // overlaps the current block's transforms.
+    float nxt[T][4];
+    if (wave < nblk) {
+        fetch(wave, nxt);
+    }
+
     __syncthreads();
 
     for (int b = wave; b < nblk; b += kWaves) {
         const int bly = b / nbx;
         const int blx = b - bly * nbx;
-        const int py0 = (by0 + bly) * S;
-        const int px0 = (bx0 + blx) * S;
-        const long long row_off = static_cast<long long>(mirror(py0 + l16 - job.oy, job.height)) * job.src_rs;
-        int col[4];
+        float cur[T][4];
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            col[s] = mirror(px0 + 4 * lg + s - job.ox, job.width);
+        for (int t = 0; t < T; t++) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                cur[t][s] = nxt[t][s];
+            }
+        }
+        if (b + kWaves < nblk) {
+            fetch(b + kWaves, nxt);
         }
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const Tin* row = frame[t] + row_off;
-            const Opnd in = make_opnd<PREC>(win[t][0] * load_sample(row + col[0]), win[t][1] * load_sample(row + col[1]),
-                                            win[t][2] * load_sample(row + col[2]), win[t][3] * load_sample(row + col[3]));
+            const Opnd in = make_opnd<PREC>(win[t][0] * cur[t][0], win[t][1] * cur[t][1], win[t][2] * cur[t][2],
+                                            win[t][3] * cur[t][3]);
             const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
             const floatx4 d1 = mm16<PREC>(in, WXo, zero);
             d2[t] = mm16<PREC>(WYo, make_opnd<PREC>(d1[0], d1[1], d1[2], d1[3]), zero);
@@ -299,7 +333,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         for (int j = 0; j < T; j++) {
             U4[j] = *reinterpret_cast<const float4*>(&wbuf[((j * 8 + mp) * 8 + mq) * 4]);
         }
-        const float gf = a.zero_mean ? wbuf[0] / a.wf0 : 0.0f;
+        const float gf = a.zero_mean ? wbuf[0] * a.inv_wf0 : 0.0f;
         float v00 = 0.0f, v01 = 0.0f, v10 = 0.0f, v11 = 0.0f;
 #pragma unroll
         for (int g = 0; g <= R; g++) {
