@@ -81,6 +81,9 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "split_bf16"], default="fp32")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
+    ap.add_argument("--workload", choices=["fp32_1080p", "precision4k", "diff"], default="fp32_1080p",
+                    help="fp32_1080p: the headline line; precision4k: 4K 10-bit FP32 vs split-BF16 with "
+                         "errors vs FP64; diff: forward vs forward+backward of the differentiable filter")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path has no multi-GPU form here (use --gpus 1)",
@@ -98,6 +101,11 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": n, "dispatches": 3}), flush=True)
         return
+
+    if args.workload == "precision4k":
+        return bench_precision4k(args)
+    if args.workload == "diff":
+        return bench_diff(args)
 
     # Whole clip in one fused launch: scratch for 240 frames of tile regions.
     os.environ.setdefault("DFT2_SCRATCH_MB", "8192")
@@ -177,6 +185,105 @@ def main():
         print(f"[bench] CPU baseline on {n_cpu} frames, {threads} threads", file=sys.stderr, flush=True)
         result["cpu_baseline"] = cpu_baseline(planes_cpu, params, threads)
     print(json.dumps(result), flush=True)
+
+
+def _time_steps(fn, steps, warmup, stream):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def bench_precision4k(args):
+    """4K YUV420P10 (3840x2160 + 2x1920x1080, 120 frames), ftype 1, tbsize 5, sigma curve
+    0.0:4 0.5:8 1.0:16: FP32 vs split-BF16 throughput, and both forms' error against the FP64
+    oracle on a 640x360x8 clip of the same generator and parameters."""
+    os.environ.setdefault("DFT2_SCRATCH_MB", "8192")
+    import numpy as np
+    import torch
+
+    from dfttest2_amd import synth
+    from dfttest2_amd.dfttest2 import Backend, get_plan, plane_job
+    from dfttest2_amd.params import dfttest_params
+    from oracle import dfttest_np
+
+    w, h, frames = 3840, 2160, 120
+    params = dfttest_params(ftype=1, tbsize=5, slocation=[0.0, 4, 0.5, 8, 1.0, 16])
+    dev = torch.device("cuda", 0)
+    print(f"[bench] generating {frames} frames of {w}x{h} YUV420P10", file=sys.stderr, flush=True)
+    planes = synth.yuv420_planes(frames, w, h, seed=2, bits=10, sd_lsb=12.0, device=dev)
+    outs = [torch.empty_like(p) for p in planes]
+    jobs = [plane_job(p, o) for p, o in zip(planes, outs)]
+    stream = torch.cuda.current_stream(dev)
+    res = {}
+    for prec in ("fp32", "split_bf16"):
+        plan = get_plan(params, "int", 10, Backend.MI355X(precision=prec))
+        ms = _time_steps(lambda: plan.process_batch(jobs, stream.cuda_stream), args.steps, args.warmup, stream)
+        res[prec] = {"ms_per_step": round(ms, 3), "fps": round(frames / (ms / 1e3), 1),
+                     "mpx_s": round(w * h * frames / (ms / 1e3) / 1e6, 1)}
+        print(f"[bench] {prec}: {ms:.2f} ms per {frames} frames", file=sys.stderr, flush=True)
+    # error yardstick: FP64 oracle on a smaller clip of the same generator and parameters
+    small = [pl.numpy() for pl in synth.yuv420_planes(8, 640, 360, seed=3, bits=10, sd_lsb=12.0)]
+    for prec in ("fp32", "split_bf16"):
+        plan = get_plan(params, "int", 10, Backend.MI355X(precision=prec))
+        worst, flips, total = 0, 0, 0
+        for pl in small:
+            x = torch.from_numpy(pl).to(dev)
+            o = torch.empty_like(x)
+            plan.process_batch([plane_job(x, o)], stream.cuda_stream)
+            torch.cuda.synchronize()
+            got = o.cpu().numpy().astype(np.int64)
+            ref = dfttest_np.process_plane(pl, params, 10, False).astype(np.int64)
+            worst = max(worst, int(np.abs(got - ref).max()))
+            flips += int((got != ref).sum())
+            total += got.size
+        res[prec].update({"max_lsb_vs_fp64": worst, "samples_differing_vs_fp64": flips, "samples": total})
+    res["speedup_split_bf16"] = round(res["fp32"]["ms_per_step"] / res["split_bf16"]["ms_per_step"], 3)
+    print(json.dumps({"metric": METRIC, "workload": "precision4k: YUV420P10 3840x2160 + 2x1920x1080, 120 frames, "
+                      "ftype 1, tbsize 5, slocation 0.0:4 0.5:8 1.0:16", "unit": "fps", "results": res}), flush=True)
+
+
+def bench_diff(args):
+    """8 float32 clips of 256x256x16 frames: noisy = clean + N(0, 0.05), sigma [3, 16, 9] = 8
+    with requires_grad, loss = MSE(DFTTest(noisy), clean): forward vs forward+backward."""
+    import torch
+
+    from dfttest2_amd import synth
+    from dfttest2_amd.autograd import dfttest_diff, diff_params
+
+    dev = torch.device("cuda", 0)
+    p = diff_params(tbsize=3)
+    clean = torch.stack([synth.clean_plane(16, 256, 256, 50 + i, device=dev) for i in range(8)])
+    noisy = clean + torch.stack([synth.noise((16, 256, 256), 0.05, 60 + i, device=dev) for i in range(8)])
+    x = noisy.clone().requires_grad_(True)
+    sigma = torch.full((3, 16, 9), 8.0, device=dev, requires_grad=True)
+    stream = torch.cuda.current_stream(dev)
+
+    def fwd():
+        with torch.no_grad():
+            dfttest_diff(x, sigma, p)
+
+    def fwd_bwd():
+        loss = torch.mean((dfttest_diff(x, sigma, p) - clean) ** 2)
+        loss.backward()
+
+    ms_f = _time_steps(fwd, args.steps, args.warmup, stream)
+    ms_fb = _time_steps(fwd_bwd, args.steps, args.warmup, stream)
+    mpx = 8 * 16 * 256 * 256 / 1e6
+    print(json.dumps({"metric": METRIC, "workload": "diff: 8 x 256x256x16 float32 clips, Wiener, MSE loss",
+                      "unit": "Mpx/s", "fwd_mpx_s": round(mpx / (ms_f / 1e3), 1),
+                      "fwd_bwd_mpx_s": round(mpx / (ms_fb / 1e3), 1),
+                      "ratio_fwd_bwd_to_fwd": round(ms_f / ms_fb, 3), "ms_fwd": round(ms_f, 3),
+                      "ms_fwd_bwd": round(ms_fb, 3)}), flush=True)
 
 
 if __name__ == "__main__":

@@ -16,6 +16,9 @@
  *   dft2_process_batch  <- the same per-frame work over whole clips resident in device
  *                          memory, many clips/planes per launch (no reference
  *                          counterpart; the reference launches per frame and plane)
+ *   dft2_process_batch_sigma, dft2_backward_batch
+ *                       <- no reference counterpart: the differentiable form of the
+ *                          filter (sigma as a device array) and its backward pass
  *   dft2_rdft           <- RDFT, cpu_source/source.cpp:269-340 (registered :390-398)
  *   dft2_version        <- Version, cpu_source/source.cpp:342-354
  */
@@ -77,6 +80,25 @@ typedef struct dft2_job {
     int frame_end;
 } dft2_job;
 
+/* One plane of one clip for the backward pass (float32 planes in device memory).
+ * grad_in receives d loss / d src and is accumulated with +=, so the caller zeroes it. */
+typedef struct dft2_grad_job {
+    const void* src;       /* forward input planes */
+    const void* grad_out;  /* d loss / d output planes */
+    void* grad_in;         /* d loss / d input planes (accumulated) */
+    long long src_frame_pitch;
+    long long src_stride;
+    long long go_frame_pitch;
+    long long go_stride;
+    long long gi_frame_pitch;
+    long long gi_stride;
+    int width;
+    int height;
+    int n_frames;
+    int frame_begin;       /* output frames [frame_begin, frame_end) whose gradient is given */
+    int frame_end;
+} dft2_grad_job;
+
 typedef struct dft2_plan dft2_plan;
 
 /* Creates a plan on `device_id`.  sample_type / bits_per_sample give the clip format
@@ -99,6 +121,21 @@ int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, long long
 /* Filters every job (device memory) with one fused launch per frame chunk, queued on
  * `stream` (NULL = the plan's own stream, and the call waits for completion). */
 int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
+
+/* As dft2_process_batch, with sigma read from device memory: sigma_dev holds T*16*9
+ * floats [t][ky][kx] and the filter uses sigma_dev * sigma_scale (sigma_scale = the
+ * window's sum of squares for ftype 0/1 in user units, as python/dfttest2.py:329-333).
+ * Gains at kx = 0, 8 average each coefficient's own and conjugate partner's sigma.
+ * No reference counterpart: the differentiable form of the filter. */
+int dft2_process_batch_sigma(dft2_plan* plan, const dft2_job* jobs, int n_jobs, const float* sigma_dev,
+                             float sigma_scale, void* stream);
+
+/* Backward pass of dft2_process_batch_sigma for the Wiener filter (filter_type 0) on
+ * float32 planes: accumulates d loss / d src into each job's grad_in and
+ * d loss / d sigma_dev into sigma_grad_dev (T*16*9 floats, device, +=), given grad_out.
+ * Block spectra are recomputed, not stored.  No reference counterpart. */
+int dft2_backward_batch(dft2_plan* plan, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
+                        float sigma_scale, float* sigma_grad_dev, void* stream);
 
 /* FP64 real DFT of `data` viewed as `shape` (ndim 1..3), unnormalised, exponent -2*pi*i,
  * real along the last axis.  out receives prod(shape[:-1])*(shape[-1]/2+1) complex values

@@ -1,0 +1,84 @@
+"""Differentiable path on the GPU: forward of dfttest_diff against the oracle, and the
+backward kernel's gradients (input planes and sigma) against the FP64 torch autograd
+graph of the same pipeline (oracle/dfttest_torch.py, itself pinned against finite
+differences).  Bar: relative error <= 1e-3 (norm-wise)."""
+
+import sys
+
+import numpy as np
+import pytest
+
+from dfttest2_amd import synth
+from dfttest2_amd.autograd import dfttest_diff, diff_params
+from oracle import dfttest_np as onp
+from oracle import dfttest_torch as otorch
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def announce(*what):
+    print("\n[case]", *what, flush=True)
+    sys.stdout.flush()
+
+
+def rel_err(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).norm() / max(float(b.norm()), 1e-30))
+
+
+@pytest.mark.parametrize("tbsize", [1, 3, 5])
+def test_diff_forward_matches_oracle(tbsize):
+    announce("diff forward", tbsize)
+    p = diff_params(tbsize=tbsize)
+    x = synth.float_plane(5, 44, 52, 7)
+    rng = np.random.default_rng(5)
+    sigma = np.abs(rng.normal(8.0, 4.0, (p.t_len, 16, 9))).astype(np.float32)  # asymmetric
+    y = dfttest_diff(x.cuda(), torch.from_numpy(sigma).cuda(), p)
+    p.sigma = (sigma.astype(np.float64) * p.extra["wscale"]).reshape(-1)
+    ref = onp.process_plane(x.numpy(), p, 32, True)
+    assert float(np.max(np.abs(y.cpu().numpy() - ref))) <= 1e-5
+
+
+@pytest.mark.parametrize("tbsize,shape", [(3, (2, 5, 40, 44)), (1, (1, 3, 36, 36)), (5, (1, 6, 34, 40))])
+def test_gradients_match_fp64_autograd(tbsize, shape):
+    announce("diff backward", tbsize, shape)
+    p = diff_params(tbsize=tbsize)
+    b, n, h, w = shape
+    clean = torch.stack([synth.clean_plane(n, h, w, 30 + i) for i in range(b)])
+    noisy = clean + torch.stack([synth.noise((n, h, w), 0.05, 40 + i) for i in range(b)])
+    rng = np.random.default_rng(9)
+    sigma0 = torch.from_numpy(np.abs(rng.normal(8.0, 3.0, (p.t_len, 16, 9))).astype(np.float32))
+
+    xg = noisy.cuda().requires_grad_(True)
+    sg = sigma0.cuda().requires_grad_(True)
+    loss = torch.mean((dfttest_diff(xg, sg, p) - clean.cuda()) ** 2)
+    loss.backward()
+
+    xr = noisy.double().requires_grad_(True)
+    sr = sigma0.double().requires_grad_(True)
+    wscale = p.extra["wscale"]
+    yr = torch.stack([otorch.filter_plane(xr[i], sr * wscale, p) for i in range(b)])
+    loss_r = torch.mean((yr - clean.double()) ** 2)
+    loss_r.backward()
+
+    assert float(loss) == pytest.approx(float(loss_r), rel=1e-5)
+    assert rel_err(xg.grad, xr.grad) <= 1e-3
+    assert rel_err(sg.grad, sr.grad) <= 1e-3
+
+
+def test_backward_requires_wiener_float_plan():
+    announce("backward argument checks")
+    from dfttest2_amd import _lib
+    from dfttest2_amd.dfttest2 import Backend, get_plan
+    from dfttest2_amd.params import dfttest_params
+
+    plan = get_plan(dfttest_params(ftype=1), "float", 32, Backend.MI355X())
+    x = torch.zeros((3, 40, 40), device="cuda")
+    s = torch.zeros((3, 16, 9), device="cuda")
+    job = _lib.Dft2GradJob(src=x.data_ptr(), grad_out=x.data_ptr(), grad_in=x.data_ptr(), src_frame_pitch=6400,
+                           src_stride=160, go_frame_pitch=6400, go_stride=160, gi_frame_pitch=6400, gi_stride=160,
+                           width=40, height=40, n_frames=3, frame_begin=0, frame_end=3)
+    with pytest.raises(_lib.Dft2Error, match="filter_type 0"):
+        plan.backward_batch([job], s.data_ptr(), 1.0, s.data_ptr())

@@ -249,8 +249,59 @@ int check_mirror(int size, int step, const char* what) {
     return DFT2_OK;
 }
 
+// Fills the launch arguments common to every kernel of a plan.
+LaunchArgs make_args(const dft2_plan* p) {
+    LaunchArgs a{};
+    a.consts = p->d_consts;
+    a.block_step = p->block_step;
+    a.tile_bw = p->tile_bw;
+    a.tile_bh = p->tile_bh;
+    a.region_w = p->region_w;
+    a.region_h = p->region_h;
+    a.inv_wf0 = 1.0f / p->wf0;
+    a.sigma2 = p->sigma2;
+    a.pmin = p->pmin;
+    a.pmax = p->pmax;
+    a.filter_type = p->filter_type;
+    a.zero_mean = p->zero_mean;
+    a.asym = p->asym;
+    a.out_scale = p->out_scale;
+    a.peak = p->peak;
+    a.sigma_dev = nullptr;
+    a.sigma_scale = 1.0f;
+    a.sigma_grad = nullptr;
+    return a;
+}
+
+// Stages `table_bytes` of job descriptors through the slot's pinned ring into d_jobs.
+int upload_jobs(Slot* slot, const JobDev* src_table, size_t table_bytes, hipStream_t stream) {
+    if (slot->d_jobs_cap < table_bytes) {
+        HIP_TRY(hipStreamSynchronize(stream));
+        int rc = ensure_device(reinterpret_cast<void**>(&slot->d_jobs), &slot->d_jobs_cap, table_bytes);
+        if (rc) {
+            return rc;
+        }
+    }
+    const int ring = slot->ring_pos;
+    slot->ring_pos = (slot->ring_pos + 1) % kRing;
+    if (slot->h_jobs_done[ring]) {
+        HIP_TRY(hipEventSynchronize(slot->h_jobs_done[ring]));
+    } else {
+        HIP_TRY(hipEventCreateWithFlags(&slot->h_jobs_done[ring], hipEventDisableTiming));
+    }
+    int rc = ensure_host(reinterpret_cast<void**>(&slot->h_jobs[ring]), &slot->h_jobs_cap[ring], table_bytes);
+    if (rc) {
+        return rc;
+    }
+    std::memcpy(slot->h_jobs[ring], src_table, table_bytes);
+    HIP_TRY(hipMemcpyAsync(slot->d_jobs, slot->h_jobs[ring], table_bytes, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipEventRecord(slot->h_jobs_done[ring], stream));
+    return DFT2_OK;
+}
+
 // Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
-int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream) {
+int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
+             const float* sigma_dev = nullptr, float sigma_scale = 1.0f) {
     const int n_jobs = static_cast<int>(jobs_in.size());
     const long long region = static_cast<long long>(p->region_w) * p->region_h;
     long long per_frame_items = 0;
@@ -273,26 +324,9 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
     }
     const size_t table_bytes = sizeof(JobDev) * static_cast<size_t>(n_jobs) * n_chunks;
     // Device job table: reused across calls on this slot; stream order (or the wait on
-    // the slot's previous completion below) keeps earlier kernels from seeing new data.
-    if (slot->d_jobs_cap < table_bytes) {
-        HIP_TRY(hipStreamSynchronize(stream));
-        rc = ensure_device(reinterpret_cast<void**>(&slot->d_jobs), &slot->d_jobs_cap, table_bytes);
-        if (rc) {
-            return rc;
-        }
-    }
-    const int ring = slot->ring_pos;
-    slot->ring_pos = (slot->ring_pos + 1) % kRing;
-    if (slot->h_jobs_done[ring]) {
-        HIP_TRY(hipEventSynchronize(slot->h_jobs_done[ring]));
-    } else {
-        HIP_TRY(hipEventCreateWithFlags(&slot->h_jobs_done[ring], hipEventDisableTiming));
-    }
-    rc = ensure_host(reinterpret_cast<void**>(&slot->h_jobs[ring]), &slot->h_jobs_cap[ring], table_bytes);
-    if (rc) {
-        return rc;
-    }
-    JobDev* h = slot->h_jobs[ring];
+    // the slot's previous completion) keeps earlier kernels from seeing new data.
+    std::vector<JobDev> table(static_cast<size_t>(n_jobs) * n_chunks);
+    JobDev* h = table.data();
     std::vector<long long> chunk_items(n_chunks), chunk_frames(n_chunks);
     std::vector<int> chunk_maxpix(n_chunks);
     for (int c = 0; c < n_chunks; c++) {
@@ -317,27 +351,19 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         chunk_frames[c] = jf;
         chunk_maxpix[c] = maxpix;
     }
-    HIP_TRY(hipMemcpyAsync(slot->d_jobs, h, table_bytes, hipMemcpyHostToDevice, stream));
-    HIP_TRY(hipEventRecord(slot->h_jobs_done[ring], stream));
+    rc = upload_jobs(slot, h, table_bytes, stream);
+    if (rc) {
+        return rc;
+    }
 
-    LaunchArgs a{};
-    a.consts = p->d_consts;
+    LaunchArgs a = make_args(p);
     a.n_jobs = n_jobs;
     a.scratch = slot->scratch;
-    a.block_step = p->block_step;
-    a.tile_bw = p->tile_bw;
-    a.tile_bh = p->tile_bh;
-    a.region_w = p->region_w;
-    a.region_h = p->region_h;
-    a.inv_wf0 = 1.0f / p->wf0;
-    a.sigma2 = p->sigma2;
-    a.pmin = p->pmin;
-    a.pmax = p->pmax;
-    a.filter_type = p->filter_type;
-    a.zero_mean = p->zero_mean;
-    a.asym = p->asym;
-    a.out_scale = p->out_scale;
-    a.peak = p->peak;
+    if (sigma_dev) {
+        a.sigma_dev = sigma_dev;
+        a.sigma_scale = sigma_scale;
+        a.asym = 1;
+    }
     for (int c = 0; c < n_chunks; c++) {
         if (chunk_items[c] == 0) {
             continue;
@@ -581,6 +607,11 @@ int dft2_plan_destroy(dft2_plan* p) {
 }
 
 int dft2_process_batch(dft2_plan* p, const dft2_job* jobs, int n_jobs, void* stream) {
+    return dft2_process_batch_sigma(p, jobs, n_jobs, nullptr, 1.0f, stream);
+}
+
+int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, const float* sigma_dev,
+                             float sigma_scale, void* stream) {
     g_last_error.clear();
     if (!p) {
         return fail(DFT2_ERR_ARG, "plan must not be NULL");
@@ -605,10 +636,90 @@ int dft2_process_batch(dft2_plan* p, const dft2_job* jobs, int n_jobs, void* str
     if (stream) {
         HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     }
-    rc = run_jobs(p, slot, jd, st);
+    rc = run_jobs(p, slot, jd, st, sigma_dev, sigma_scale);
     if (rc) {
         return rc;
     }
+    HIP_TRY(hipEventRecord(slot->done, st));
+    if (!stream) {
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return DFT2_OK;
+}
+
+int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
+                        float sigma_scale, float* sigma_grad_dev, void* stream) {
+    g_last_error.clear();
+    if (!p || !sigma_dev || !sigma_grad_dev) {
+        return fail(DFT2_ERR_ARG, "plan, sigma_dev and sigma_grad_dev must not be NULL");
+    }
+    if (p->sample_code != kF32 || p->filter_type != 0 || p->precision != DFT2_PRECISION_FP32) {
+        return fail(DFT2_ERR_ARG, "the backward pass needs a float32 plan with filter_type 0 and FP32 precision");
+    }
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
+        return fail(DFT2_ERR_ARG, "invalid job list");
+    }
+    std::vector<JobDev> jd(n_jobs);
+    long long items = 0;
+    for (int i = 0; i < n_jobs; i++) {
+        const dft2_grad_job& g = jobs[i];
+        if (!g.grad_out || !g.grad_in) {
+            return fail(DFT2_ERR_ARG, "grad_out and grad_in must not be NULL");
+        }
+        dft2_job uj{};
+        uj.src = g.src;
+        uj.dst = g.grad_in;
+        uj.src_frame_pitch = g.src_frame_pitch;
+        uj.src_stride = g.src_stride;
+        uj.dst_frame_pitch = g.gi_frame_pitch;
+        uj.dst_stride = g.gi_stride;
+        uj.width = g.width;
+        uj.height = g.height;
+        uj.n_frames = g.n_frames;
+        uj.frame_begin = g.frame_begin;
+        uj.frame_end = g.frame_end;
+        int rc = make_job(p, uj, jd[i]);
+        if (rc) {
+            return rc;
+        }
+        if (g.go_stride < static_cast<long long>(g.width) * 4 || g.go_stride % 4 || g.go_frame_pitch % 4 ||
+            g.gi_frame_pitch % 4) {
+            return fail(DFT2_ERR_ARG, "invalid gradient strides");
+        }
+        jd[i].grad_out = g.grad_out;
+        jd[i].grad_in = g.grad_in;
+        jd[i].go_fp = g.go_frame_pitch / 4;
+        jd[i].go_rs = g.go_stride / 4;
+        jd[i].gi_fp = g.gi_frame_pitch / 4;
+        jd[i].gi_rs = g.gi_stride / 4;
+        jd[i].item_begin = items;
+        items += static_cast<long long>(jd[i].f_count) * jd[i].tiles_x * jd[i].tiles_y;
+    }
+    if (items == 0) {
+        return DFT2_OK;
+    }
+    int rc = select_device(p);
+    if (rc) {
+        return rc;
+    }
+    Slot* slot = acquire_slot(p);
+    SlotGuard guard{p, slot};
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : slot->stream;
+    if (stream) {
+        HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    }
+    rc = upload_jobs(slot, jd.data(), sizeof(JobDev) * jd.size(), st);
+    if (rc) {
+        return rc;
+    }
+    LaunchArgs a = make_args(p);
+    a.jobs = slot->d_jobs;
+    a.n_jobs = n_jobs;
+    a.sigma_dev = sigma_dev;
+    a.sigma_scale = sigma_scale;
+    a.sigma_grad = sigma_grad_dev;
+    a.asym = 1;
+    HIP_TRY(launch_backward(a, p->radius, items, backward_lds_bytes(p->radius), st));
     HIP_TRY(hipEventRecord(slot->done, st));
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));

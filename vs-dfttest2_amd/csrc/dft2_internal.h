@@ -48,6 +48,11 @@ struct JobDev {
     long long frame_begin;      // first job-frame index (finalize kernel) of this job
     int use_fptr;               // 1: source frame f is fptr[f] (f < 7) instead of src + f*src_fp
     const void* fptr[7];
+    // Backward pass only: gradient of the output (same geometry as dst) and gradient of
+    // the input (same geometry as src, accumulated with +=), pitches in samples.
+    const void* grad_out;
+    void* grad_in;
+    long long go_fp, go_rs, gi_fp, gi_rs;
 };
 
 struct LaunchArgs {
@@ -65,6 +70,11 @@ struct LaunchArgs {
     int asym;                   // sigma not symmetric under (kt,ky) -> (-kt,-ky) at kx = 0, 8
     float out_scale;            // sample scale (2^(8-bits) for int, 255 for float)
     int peak;                   // 2^bits - 1
+    // Optional device sigma [T][16][9] (times sigma_scale) replacing the plan's sigma in
+    // the gain table; used by the differentiable path, where sigma is a tensor.
+    const float* sigma_dev;
+    float sigma_scale;
+    float* sigma_grad;          // backward: d loss / d sigma_dev, accumulated with +=
 };
 
 // Sample type codes.
@@ -74,6 +84,8 @@ hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, int pr
                         size_t lds_bytes, hipStream_t stream);
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,
                            hipStream_t stream);
+hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds_bytes, hipStream_t stream);
 size_t fused_lds_bytes(int radius, int region_w, int region_h);
+size_t backward_lds_bytes(int radius);
 
 } // namespace dft2

@@ -160,6 +160,47 @@ __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, floa
     im = xi * g + gf * e.w;
 }
 
+// Coefficient (kt, ky, kx) owned by middle-stage lane `lane` at temporal pair g and slot
+// (A of evaluation 0, B of 0, A of 1, B of 1); see the middle stage of fused_kernel.
+__device__ __forceinline__ int3 slot_position(int g, int slot, int lane, int t_len) {
+    const int pp = lane >> 3, q = lane & 7;
+    const int e = slot >> 1, k = slot & 1;
+    const int ky = pp != 0 ? (e == 0 ? pp : (16 - pp) & 15) : (e == 0 ? 0 : 8);
+    if (q != 0) {
+        return k == 0 ? make_int3(g, ky, q) : make_int3((t_len - g) % t_len, (16 - ky) & 15, q);
+    }
+    return make_int3(g, ky, k == 0 ? 0 : 8);
+}
+
+// Conjugate partner whose sigma is averaged with `p` at kx = 0, 8 (itself elsewhere).
+__device__ __forceinline__ int3 partner_position(int3 p, int t_len) {
+    if (p.z == 0 || p.z == 8) {
+        return make_int3((t_len - p.x) % t_len, (16 - p.y) & 15, p.z);
+    }
+    return p;
+}
+
+__device__ __forceinline__ int sigma_index(int3 p) {
+    return (p.x * 16 + p.y) * 9 + p.z;
+}
+
+// Gain table into LDS: the plan's table, with sigma taken from a.sigma_dev when given.
+template <int R>
+__device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int tid, int n_threads) {
+    constexpr int T = 2 * R + 1;
+    const float4* gtab = reinterpret_cast<const float4*>(a.consts + kOffTAB);
+    for (int i = tid; i < (R + 1) * 4 * 64; i += n_threads) {
+        float4 e = gtab[i];
+        if (a.sigma_dev) {
+            const int lane = i & 63, slot = (i >> 6) & 3, g = i >> 8;
+            const int3 pos = slot_position(g, slot, lane, T);
+            e.x = a.sigma_dev[sigma_index(pos)] * a.sigma_scale;
+            e.y = a.sigma_dev[sigma_index(partner_position(pos, T))] * a.sigma_scale;
+        }
+        tab[i] = e;
+    }
+}
+
 template <int R, typename Tin, int FTM, int PREC>
 __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
@@ -202,10 +243,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     for (int i = tid; i < rows_used * rw; i += kThreads) {
         acc[i] = 0.0f;
     }
-    const float4* gtab = reinterpret_cast<const float4*>(a.consts + kOffTAB);
-    for (int i = tid; i < (R + 1) * 4 * 64; i += kThreads) {
-        tab[i] = gtab[i];
-    }
+    fill_tab<R>(tab, a, tid, kThreads);
 
     // ---- per-lane constant operands ----
     float wx[4], wy[4], gx[4], gy[4], syn[4];
@@ -432,6 +470,360 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Backward pass (Wiener gain, float32 planes): gradients of a loss with respect to the
+// input planes and to the sigma array, given the gradient of the output planes.
+// It recomputes each block's spectrum (forward transform + middle stage) instead of
+// storing it, then applies the adjoint of every stage in reverse:
+//   output gradient -> x synthesis window / scale -> adjoint inverse DFT (two MFMA)
+//   -> adjoint middle stage (mean re-add, gain, zero-mean, complex<->real basis maps)
+//   -> adjoint temporal basis -> adjoint forward DFT (two MFMA) -> x analysis window
+//   -> scatter-add into the input gradient through the same mirror / temporal clamp.
+// With sigma taken from a device array the gain is always the average of a coefficient's
+// own gain and its conjugate partner's (kx = 0, 8), so each sigma entry gets its share.
+
+// Wiener gain g = max((psd - s)/(psd + eps), 0) and its derivatives where g > 0:
+// dg/dpsd = (s + eps)/(psd + eps)^2, dg/ds = -1/(psd + eps); both 0 elsewhere.
+__device__ __forceinline__ float3 wiener_d(float psd, float s) {
+    const float r = __builtin_amdgcn_rcpf(psd + 1e-15f);
+    const float g = (psd - s) * r;
+    return g > 0.0f ? make_float3(g, (s + 1e-15f) * r * r, -r) : make_float3(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+    }
+    return v;
+}
+
+// Adjoint of one gain application: x = A - gf*W, y = g(|x|^2) x + gf*W.  (dyr, dyi) in,
+// gradient w.r.t. A out in (dyr, dyi); accumulates d/dsigma (own, partner) and d/dgf.
+__device__ __forceinline__ void gain_adjoint(float ar, float ai, float& dyr, float& dyi, float4 e, float gf,
+                                             float2& dsig, float& dgf) {
+    const float xr = ar - gf * e.z;
+    const float xi = ai - gf * e.w;
+    const float psd = xr * xr + xi * xi;
+    const float3 d1 = wiener_d(psd, e.x);
+    const float3 d2 = wiener_d(psd, e.y);
+    const float g = 0.5f * (d1.x + d2.x);
+    const float h = 0.5f * (d1.y + d2.y);
+    const float sdot = dyr * xr + dyi * xi;
+    const float dxr = g * dyr + 2.0f * h * sdot * xr;
+    const float dxi = g * dyi + 2.0f * h * sdot * xi;
+    dsig.x += 0.5f * sdot * d1.z;
+    dsig.y += 0.5f * sdot * d2.z;
+    dgf += (dyr - dxr) * e.z + (dyi - dxi) * e.w;
+    dyr = dxr;
+    dyi = dxi;
+}
+
+template <int R>
+__global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
+    constexpr int T = 2 * R + 1;
+    constexpr int kWbuf = T * 256 + 256;
+    constexpr int kSigPad = (T * 144 + 3) / 4 * 4;
+    extern __shared__ float smem[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int l16 = lane & 15;
+    const int lg = lane >> 4;
+
+    const long long item = blockIdx.x;
+    int jb = 0;
+    while (jb + 1 < a.n_jobs && a.jobs[jb + 1].item_begin <= item) {
+        ++jb;
+    }
+    const JobDev& job = a.jobs[jb];
+    const int tiles = job.tiles_x * job.tiles_y;
+    const long long local = item - job.item_begin;
+    const int f_local = static_cast<int>(local / tiles);
+    const int tile = static_cast<int>(local - static_cast<long long>(f_local) * tiles);
+    const int ty = tile / job.tiles_x;
+    const int tx = tile - ty * job.tiles_x;
+    const int n = job.f_begin + f_local;
+    const int S = a.block_step;
+    const int by0 = ty * a.tile_bh;
+    const int bx0 = tx * a.tile_bw;
+    const int nby = min(a.tile_bh, job.nby - by0);
+    const int nbx = min(a.tile_bw, job.nbx - bx0);
+    const int nblk = nby * nbx;
+
+    float4* tab = reinterpret_cast<float4*>(smem);
+    float* dsig_lds = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
+    float* wbuf = dsig_lds + kSigPad + wave * kWbuf;
+    float* vbuf = wbuf + T * 256;
+    fill_tab<R>(tab, a, tid, kThreads);
+    for (int i = tid; i < T * 144; i += kThreads) {
+        dsig_lds[i] = 0.0f;
+    }
+
+    float wx[4], wy[4], gxT[4], gyT[4], wyT[4], wxT[4], synA[4];
+    float win[T][4];
+    const float inv_scale = 1.0f / a.out_scale;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        wx[s] = a.consts[kOffWX + (4 * lg + s) * 16 + l16];
+        wy[s] = a.consts[kOffWY + l16 * 16 + 4 * lg + s];
+        gxT[s] = a.consts[kOffGX + l16 * 16 + 4 * lg + s];
+        gyT[s] = a.consts[kOffGY + (4 * lg + s) * 16 + l16];
+        wyT[s] = a.consts[kOffWY + (4 * lg + s) * 16 + l16];
+        wxT[s] = a.consts[kOffWX + l16 * 16 + 4 * lg + s];
+        synA[s] = a.consts[kOffSYN + l16 * 16 + 4 * lg + s] * inv_scale;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
+        }
+    }
+    const Opnd WXo = make_opnd<0>(wx[0], wx[1], wx[2], wx[3]);
+    const Opnd WYo = make_opnd<0>(wy[0], wy[1], wy[2], wy[3]);
+    const Opnd GXTo = make_opnd<0>(gxT[0], gxT[1], gxT[2], gxT[3]);
+    const Opnd GYTo = make_opnd<0>(gyT[0], gyT[1], gyT[2], gyT[3]);
+    const Opnd WYTo = make_opnd<0>(wyT[0], wyT[1], wyT[2], wyT[3]);
+    const Opnd WXTo = make_opnd<0>(wxT[0], wxT[1], wxT[2], wxT[3]);
+    float wt[T][T];
+    float ct[T];
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+        ct[j] = a.consts[kOffCT + j];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            wt[j][t] = a.consts[kOffWT + j * T + t];
+        }
+    }
+
+    gptr_t<float> frame[T];
+    float* gin[T];
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        const int fi = min(max(n - R + t, 0), job.n_frames - 1);
+        frame[t] = to_global(reinterpret_cast<const float*>(job.src) + static_cast<long long>(fi) * job.src_fp);
+        gin[t] = reinterpret_cast<float*>(job.grad_in) + static_cast<long long>(fi) * job.gi_fp;
+    }
+    const gptr_t<float> gout = to_global(reinterpret_cast<const float*>(job.grad_out) +
+                                         static_cast<long long>(n) * job.go_fp);
+
+    const int mp = lane >> 3;
+    const int mq = lane & 7;
+    const bool ey = mp != 0;
+    const bool ex = mq != 0;
+    const int wq = l16 & 7;
+    const int wc = l16 >> 3;
+    float2 dsg[R + 1][4];
+#pragma unroll
+    for (int g = 0; g <= R; g++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            dsg[g][k] = make_float2(0.0f, 0.0f);
+        }
+    }
+
+    __syncthreads();
+
+    for (int b = wave; b < nblk; b += kWaves) {
+        const int bly = b / nbx;
+        const int blx = b - bly * nbx;
+        const int py = (by0 + bly) * S + l16;
+        const long long roff = static_cast<long long>(mirror(py - job.oy, job.height)) * job.src_rs;
+        const long long groff = static_cast<long long>(mirror(py - job.oy, job.height)) * job.gi_rs;
+        int cols[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            cols[s] = mirror((bx0 + blx) * S + 4 * lg + s - job.ox, job.width);
+        }
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+
+        // ---- recompute the forward spectrum, staged as [j][p][q][b*2+c] ----
+        floatx4 d2[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const Opnd in = make_opnd<0>(win[t][0] * frame[t][roff + cols[0]], win[t][1] * frame[t][roff + cols[1]],
+                                         win[t][2] * frame[t][roff + cols[2]], win[t][3] * frame[t][roff + cols[3]]);
+            const floatx4 d1 = mm16<0>(in, WXo, zero);
+            d2[t] = mm16<0>(WYo, make_opnd<0>(d1[0], d1[1], d1[2], d1[3]), zero);
+        }
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            floatx4 u = d2[0] * wt[j][0];
+#pragma unroll
+            for (int t = 1; t < T; t++) {
+                u += d2[t] * wt[j][t];
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * lg + i;
+                wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = u[i];
+            }
+        }
+
+        // ---- output gradient -> dV = GY^T (dO GX^T), dO = dy / scale * synthesis window ----
+        {
+            const int fy = py - job.oy;
+            const bool row_ok = fy >= 0 && fy < job.height;
+            float dov[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int fx = (bx0 + blx) * S + 4 * lg + s - job.ox;
+                dov[s] = (row_ok && fx >= 0 && fx < job.width)
+                             ? gout[static_cast<long long>(fy) * job.go_rs + fx] * synA[s]
+                             : 0.0f;
+            }
+            const floatx4 pm = mm16<0>(make_opnd<0>(dov[0], dov[1], dov[2], dov[3]), GXTo, zero);
+            const floatx4 dv = mm16<0>(GYTo, make_opnd<0>(pm[0], pm[1], pm[2], pm[3]), zero);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * lg + i;
+                vbuf[((m & 7) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = dv[i];
+            }
+        }
+        wave_lds_sync();
+
+        // ---- adjoint middle stage, one lane per (y pair, x pair) group ----
+        float4 U4[T];
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            U4[j] = *reinterpret_cast<const float4*>(&wbuf[((j * 8 + mp) * 8 + mq) * 4]);
+        }
+        const float4 dV4 = *reinterpret_cast<const float4*>(&vbuf[(mp * 8 + mq) * 4]);
+        const float gf = a.zero_mean ? wbuf[0] * a.inv_wf0 : 0.0f;
+        float dgf = 0.0f;
+        float4 dU4[T];
+#pragma unroll
+        for (int g = 0; g <= R; g++) {
+            const float4 uc = U4[g];
+            const float4 us = g ? U4[R + g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float cc0[2] = {uc.x, uc.y}, cc1[2] = {uc.z, uc.w};
+            const float ss0[2] = {us.x, us.y}, ss1[2] = {us.z, us.w};
+            float z1r[2], z1i[2], z2r[2], z2i[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                z1r[c] = cc0[c] - (ey ? ss1[c] : 0.0f);
+                z1i[c] = -(ss0[c] + (ey ? cc1[c] : 0.0f));
+                z2r[c] = ey ? (cc0[c] + ss1[c]) : cc1[c];
+                z2i[c] = ey ? (cc1[c] - ss0[c]) : -ss1[c];
+            }
+            // forward coefficients A, B of both evaluations (pre zero-mean)
+            float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float z0r = e ? z2r[0] : z1r[0];
+                const float z0i = e ? z2i[0] : z1i[0];
+                const float zxr = e ? z2r[1] : z1r[1];
+                const float zxi = e ? z2i[1] : z1i[1];
+                ar[e] = ex ? z0r + zxi : z0r;
+                ai[e] = ex ? z0i - zxr : z0i;
+                br[e] = ex ? z0r - zxi : zxr;
+                bi[e] = ex ? -z0i - zxr : zxi;
+            }
+            // dV -> d(real t,y basis) -> dP, dQ per x slot
+            const float wcg = ct[g];
+            const float wsg = g ? ct[R + g] : 0.0f;
+            const float dv0[2] = {dV4.x, dV4.y}, dv1[2] = {dV4.z, dV4.w};
+            float dzr[2][2], dzi[2][2]; // [e][c] gradients of the gained Z'
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float duc0 = wcg * dv0[c], dus0 = wsg * dv0[c];
+                const float duc1 = wcg * dv1[c], dus1 = wsg * dv1[c];
+                dzr[0][c] = ey ? 0.5f * (duc0 - dus1) : duc0;
+                dzr[1][c] = ey ? 0.5f * (duc0 + dus1) : duc1;
+                dzi[0][c] = ey ? -0.5f * (dus0 + duc1) : -dus0;
+                dzi[1][c] = ey ? 0.5f * (duc1 - dus0) : -dus1;
+            }
+            float gz1r[2], gz1i[2], gz2r[2], gz2i[2]; // gradients of Z (pre gain), [c]
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float d0r = dzr[e][0], d0i = dzi[e][0], d1r = dzr[e][1], d1i = dzi[e][1];
+                float dar = ex ? 0.5f * (d0r + d1i) : d0r;
+                float dai = ex ? 0.5f * (d0i - d1r) : d0i;
+                float dbr = ex ? 0.5f * (d0r - d1i) : d1r;
+                float dbi = ex ? 0.5f * (-d0i - d1r) : d1i;
+                gain_adjoint(ar[e], ai[e], dar, dai, tab[(g * 4 + 2 * e) * 64 + lane], gf, dsg[g][2 * e], dgf);
+                gain_adjoint(br[e], bi[e], dbr, dbi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, dsg[g][2 * e + 1],
+                             dgf);
+                const float g0r = ex ? dar + dbr : dar;
+                const float g0i = ex ? dai - dbi : dai;
+                const float g1r = ex ? -dai - dbi : dbr;
+                const float g1i = ex ? dar - dbr : dbi;
+                if (e == 0) {
+                    gz1r[0] = g0r, gz1i[0] = g0i, gz1r[1] = g1r, gz1i[1] = g1i;
+                } else {
+                    gz2r[0] = g0r, gz2i[0] = g0i, gz2r[1] = g1r, gz2i[1] = g1i;
+                }
+            }
+            float dc0[2], dc1[2], ds0[2], ds1[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                dc0[c] = ey ? gz1r[c] + gz2r[c] : gz1r[c];
+                ds1[c] = ey ? gz2r[c] - gz1r[c] : -gz2i[c];
+                ds0[c] = ey ? -gz1i[c] - gz2i[c] : -gz1i[c];
+                dc1[c] = ey ? gz2i[c] - gz1i[c] : gz2r[c];
+            }
+            dU4[g] = make_float4(dc0[0], dc0[1], dc1[0], dc1[1]);
+            if (g) {
+                dU4[R + g] = make_float4(ds0[0], ds0[1], ds1[0], ds1[1]);
+            }
+        }
+        dgf = wave_sum(dgf);
+        if (a.zero_mean && lane == 0) {
+            dU4[0].x += dgf * a.inv_wf0;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            *reinterpret_cast<float4*>(&wbuf[((j * 8 + mp) * 8 + mq) * 4]) = dU4[j];
+        }
+        wave_lds_sync();
+
+        // ---- adjoint temporal basis and forward DFT, scatter into the input gradient ----
+        floatx4 du[T];
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * lg + i;
+                du[j][i] = wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            floatx4 dd = du[0] * wt[0][t];
+#pragma unroll
+            for (int j = 1; j < T; j++) {
+                dd += du[j] * wt[j][t];
+            }
+            // dd (m in registers) used as A operand is dD2^T; R1 = dD2^T WY, db^T = WX R1.
+            const floatx4 r1 = mm16<0>(make_opnd<0>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
+            const floatx4 dbt = mm16<0>(WXTo, make_opnd<0>(r1[0], r1[1], r1[2], r1[3]), zero);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                atomicAdd(gin[t] + groff + cols[i], win[t][i] * dbt[i]);
+            }
+        }
+        wave_lds_sync();
+    }
+
+    // sigma gradient: lanes -> LDS (by coefficient) -> global
+#pragma unroll
+    for (int g = 0; g <= R; g++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int3 pos = slot_position(g, k, lane, T);
+            atomicAdd(&dsig_lds[sigma_index(pos)], dsg[g][k].x);
+            atomicAdd(&dsig_lds[sigma_index(partner_position(pos, T))], dsg[g][k].y);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < T * 144; i += kThreads) {
+        const float v = dsig_lds[i];
+        if (v != 0.0f) {
+            atomicAdd(&a.sigma_grad[i], v * a.sigma_scale);
+        }
+    }
+}
+
 template <typename Tout>
 __device__ __forceinline__ void store_sample(Tout* p, float v, const LaunchArgs& a) {
     if constexpr (sizeof(Tout) == 4) {
@@ -544,6 +936,36 @@ hipError_t launch_type(const LaunchArgs& a, int sample_code, int precision, long
 }
 
 } // namespace
+
+size_t backward_lds_bytes(int radius) {
+    const int t_len = 2 * radius + 1;
+    return sizeof(float) * (static_cast<size_t>(radius + 1) * 4 * 64 * 4 + (t_len * 144 + 3) / 4 * 4 +
+                            static_cast<size_t>(kWaves) * (t_len * 256 + 256));
+}
+
+hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds, hipStream_t stream) {
+    auto run = [&](auto kern) -> hipError_t {
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(n_items)), dim3(kThreads), lds, stream, a);
+        return hipGetLastError();
+    };
+    switch (radius) {
+    case 0:
+        return run(backward_kernel<0>);
+    case 1:
+        return run(backward_kernel<1>);
+    case 2:
+        return run(backward_kernel<2>);
+    default:
+        return run(backward_kernel<3>);
+    }
+}
 
 size_t fused_lds_bytes(int radius, int region_w, int region_h) {
     const int t_len = 2 * radius + 1;

@@ -24,6 +24,8 @@ EXPORTED_SYMBOLS = (
     "dft2_plan_destroy",
     "dft2_process_frame",
     "dft2_process_batch",
+    "dft2_process_batch_sigma",
+    "dft2_backward_batch",
     "dft2_rdft",
     "dft2_version",
     "dft2_last_error",
@@ -67,6 +69,25 @@ class Dft2Job(ctypes.Structure):
     ]
 
 
+class Dft2GradJob(ctypes.Structure):
+    _fields_ = [
+        ("src", ctypes.c_void_p),
+        ("grad_out", ctypes.c_void_p),
+        ("grad_in", ctypes.c_void_p),
+        ("src_frame_pitch", ctypes.c_longlong),
+        ("src_stride", ctypes.c_longlong),
+        ("go_frame_pitch", ctypes.c_longlong),
+        ("go_stride", ctypes.c_longlong),
+        ("gi_frame_pitch", ctypes.c_longlong),
+        ("gi_stride", ctypes.c_longlong),
+        ("width", ctypes.c_int),
+        ("height", ctypes.c_int),
+        ("n_frames", ctypes.c_int),
+        ("frame_begin", ctypes.c_int),
+        ("frame_end", ctypes.c_int),
+    ]
+
+
 class Dft2Error(RuntimeError):
     pass
 
@@ -96,6 +117,15 @@ def load():
     lib.dft2_process_frame.restype = ctypes.c_int
     lib.dft2_process_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p]
     lib.dft2_process_batch.restype = ctypes.c_int
+    lib.dft2_process_batch_sigma.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+    ]
+    lib.dft2_process_batch_sigma.restype = ctypes.c_int
+    lib.dft2_backward_batch.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(Dft2GradJob), ctypes.c_int, ctypes.c_void_p, ctypes.c_float,
+        ctypes.c_void_p, ctypes.c_void_p,
+    ]
+    lib.dft2_backward_batch.restype = ctypes.c_int
     lib.dft2_rdft.argtypes = [_dbl_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _dbl_p, ctypes.c_int]
     lib.dft2_rdft.restype = ctypes.c_int
     lib.dft2_version.argtypes = []
@@ -176,6 +206,19 @@ class Plan:
     def process_batch(self, jobs: List[Dft2Job], stream: Optional[int] = None):
         arr = (Dft2Job * len(jobs))(*jobs)
         _check(load().dft2_process_batch(self._h, arr, len(jobs), ctypes.c_void_p(stream or 0)))
+
+    def process_batch_sigma(self, jobs: List[Dft2Job], sigma_ptr: int, sigma_scale: float,
+                            stream: Optional[int] = None):
+        arr = (Dft2Job * len(jobs))(*jobs)
+        _check(load().dft2_process_batch_sigma(self._h, arr, len(jobs), ctypes.c_void_p(sigma_ptr),
+                                               ctypes.c_float(sigma_scale), ctypes.c_void_p(stream or 0)))
+
+    def backward_batch(self, jobs: List[Dft2GradJob], sigma_ptr: int, sigma_scale: float, sigma_grad_ptr: int,
+                       stream: Optional[int] = None):
+        arr = (Dft2GradJob * len(jobs))(*jobs)
+        _check(load().dft2_backward_batch(self._h, arr, len(jobs), ctypes.c_void_p(sigma_ptr),
+                                          ctypes.c_float(sigma_scale), ctypes.c_void_p(sigma_grad_ptr),
+                                          ctypes.c_void_p(stream or 0)))
 
     def process_frame(self, src_ptrs: Sequence[int], src_stride: int, dst: int, dst_stride: int, width: int,
                       height: int, memory: int, stream: Optional[int] = None):
