@@ -63,7 +63,7 @@ def test_gradients_match_fp64_autograd(tbsize, shape):
     loss_r = torch.mean((yr - clean.double()) ** 2)
     loss_r.backward()
 
-    assert float(loss) == pytest.approx(float(loss_r), rel=1e-5)
+    assert float(loss.detach()) == pytest.approx(float(loss_r.detach()), rel=1e-5)
     assert rel_err(xg.grad, xr.grad) <= 1e-3
     assert rel_err(sg.grad, sr.grad) <= 1e-3
 

@@ -86,6 +86,6 @@ hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job
                            hipStream_t stream);
 hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds_bytes, hipStream_t stream);
 size_t fused_lds_bytes(int radius, int region_w, int region_h);
-size_t backward_lds_bytes(int radius);
+size_t backward_lds_bytes(int radius, int region_w, int region_h);
 
 } // namespace dft2

@@ -153,7 +153,7 @@ __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, floa
     float xi = im - gf * e.w;
     float psd = xr * xr + xi * xi;
     float g = gain_of<FTM>(psd, e.x, a);
-    if (a.asym) {
+    if (a.asym && e.y != e.x) {  // only the kx = 0, 8 lanes of an asymmetric sigma array
         g = 0.5f * (g + gain_of<FTM>(psd, e.y, a));
     }
     re = xr * g + gf * e.z;
@@ -552,13 +552,22 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     const int nbx = min(a.tile_bw, job.nbx - bx0);
     const int nblk = nby * nbx;
 
+    // LDS: gain table | sigma gradient | input-gradient regions [T][region_h][region_w] | per-wave staging
+    const int rw = a.region_w;
+    const int rows_used = S * (nby - 1) + 16;
+    const int cols_used = S * (nbx - 1) + 16;
+    const int region = a.region_h * rw;
     float4* tab = reinterpret_cast<float4*>(smem);
     float* dsig_lds = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
-    float* wbuf = dsig_lds + kSigPad + wave * kWbuf;
+    float* gacc = dsig_lds + kSigPad;
+    float* wbuf = gacc + T * region + wave * kWbuf;
     float* vbuf = wbuf + T * 256;
     fill_tab<R>(tab, a, tid, kThreads);
     for (int i = tid; i < T * 144; i += kThreads) {
         dsig_lds[i] = 0.0f;
+    }
+    for (int i = tid; i < T * region; i += kThreads) {
+        gacc[i] = 0.0f;
     }
 
     float wx[4], wy[4], gxT[4], gyT[4], wyT[4], wxT[4], synA[4];
@@ -628,7 +637,6 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         const int blx = b - bly * nbx;
         const int py = (by0 + bly) * S + l16;
         const long long roff = static_cast<long long>(mirror(py - job.oy, job.height)) * job.src_rs;
-        const long long groff = static_cast<long long>(mirror(py - job.oy, job.height)) * job.gi_rs;
         int cols[4];
 #pragma unroll
         for (int s = 0; s < 4; s++) {
@@ -797,12 +805,28 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
             // dd (m in registers) used as A operand is dD2^T; R1 = dD2^T WY, db^T = WX R1.
             const floatx4 r1 = mm16<0>(make_opnd<0>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
             const floatx4 dbt = mm16<0>(WXTo, make_opnd<0>(r1[0], r1[1], r1[2], r1[3]), zero);
+            float* arow = gacc + t * region + (bly * S + l16) * rw + blx * S + 4 * lg;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                atomicAdd(gin[t] + groff + cols[i], win[t][i] * dbt[i]);
+                atomicAdd(arow + i, win[t][i] * dbt[i]);
             }
         }
         wave_lds_sync();
+    }
+
+    __syncthreads();
+    // input gradient: padded tile regions -> source samples (mirror, temporal clamp)
+    for (int t = 0; t < T; t++) {
+        for (int i = tid; i < rows_used * rw; i += kThreads) {
+            const int ry = i / rw;
+            const int rx = i - ry * rw;
+            const float v = gacc[t * region + i];
+            if (rx < cols_used && v != 0.0f) {
+                const int fy = mirror(by0 * S + ry - job.oy, job.height);
+                const int fx = mirror(bx0 * S + rx - job.ox, job.width);
+                atomicAdd(gin[t] + static_cast<long long>(fy) * job.gi_rs + fx, v);
+            }
+        }
     }
 
     // sigma gradient: lanes -> LDS (by coefficient) -> global
@@ -937,9 +961,10 @@ hipError_t launch_type(const LaunchArgs& a, int sample_code, int precision, long
 
 } // namespace
 
-size_t backward_lds_bytes(int radius) {
+size_t backward_lds_bytes(int radius, int region_w, int region_h) {
     const int t_len = 2 * radius + 1;
     return sizeof(float) * (static_cast<size_t>(radius + 1) * 4 * 64 * 4 + (t_len * 144 + 3) / 4 * 4 +
+                            static_cast<size_t>(t_len) * region_w * region_h +
                             static_cast<size_t>(kWaves) * (t_len * 256 + 256));
 }
 
