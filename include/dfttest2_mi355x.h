@@ -19,6 +19,10 @@
  *   dft2_process_batch_sigma, dft2_backward_batch
  *                       <- no reference counterpart: the differentiable form of the
  *                          filter (sigma as a device array) and its backward pass
+ *   dft2_batch_*        <- no reference counterpart: a fixed set of clips of uneven
+ *                          size set up once (job table, scratch, autotuned tile shape)
+ *                          and run as one captured hipGraph launch; replaces the
+ *                          reference's per-frame launch pair (hiprtc_source/source.cpp:575-640)
  *   dft2_rdft           <- RDFT, cpu_source/source.cpp:269-340 (registered :390-398)
  *   dft2_version        <- Version, cpu_source/source.cpp:342-354
  */
@@ -26,6 +30,12 @@
 #define DFTTEST2_MI355X_H
 
 #include <stddef.h>
+
+#if defined(__GNUC__)
+#define DFT2_API __attribute__((visibility("default")))
+#else
+#define DFT2_API
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -104,49 +114,104 @@ typedef struct dft2_plan dft2_plan;
 /* Creates a plan on `device_id`.  sample_type / bits_per_sample give the clip format
  * (integer 8..16 bit, or float 32).  num_streams >= 1 concurrent calls are admitted,
  * each with its own stream and scratch (the reference's num_streams). */
-int dft2_plan_create(const dft2_params* params, int sample_type, int bits_per_sample, int device_id,
+DFT2_API int dft2_plan_create(const dft2_params* params, int sample_type, int bits_per_sample, int device_id,
                      int num_streams, int precision, dft2_plan** out_plan);
 
 /* Releases the plan and its device memory. NULL is accepted. */
-int dft2_plan_destroy(dft2_plan* plan);
+DFT2_API int dft2_plan_destroy(dft2_plan* plan);
 
 /* Filters one output frame of one plane.  src_frames holds the T = 2*radius+1 source
  * planes in temporal order (already clamped by the caller, as GetFrame gathers them).
  * memory = 0: device pointers, work is queued on `stream` (hipStream_t; NULL = the
  * plan's own stream, and the call waits for completion).  memory = 1: host pointers;
  * the call stages through device memory and returns when dst is written. */
-int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, long long src_stride, void* dst,
+DFT2_API int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, long long src_stride, void* dst,
                        long long dst_stride, int width, int height, int memory, void* stream);
 
 /* Filters every job (device memory) with one fused launch per frame chunk, queued on
  * `stream` (NULL = the plan's own stream, and the call waits for completion). */
-int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
+DFT2_API int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
 
 /* As dft2_process_batch, with sigma read from device memory: sigma_dev holds T*16*9
  * floats [t][ky][kx] and the filter uses sigma_dev * sigma_scale (sigma_scale = the
  * window's sum of squares for ftype 0/1 in user units, as python/dfttest2.py:329-333).
  * Gains at kx = 0, 8 average each coefficient's own and conjugate partner's sigma.
  * No reference counterpart: the differentiable form of the filter. */
-int dft2_process_batch_sigma(dft2_plan* plan, const dft2_job* jobs, int n_jobs, const float* sigma_dev,
+DFT2_API int dft2_process_batch_sigma(dft2_plan* plan, const dft2_job* jobs, int n_jobs, const float* sigma_dev,
                              float sigma_scale, void* stream);
 
 /* Backward pass of dft2_process_batch_sigma for the Wiener filter (filter_type 0) on
  * float32 planes: accumulates d loss / d src into each job's grad_in and
  * d loss / d sigma_dev into sigma_grad_dev (T*16*9 floats, device, +=), given grad_out.
  * Block spectra are recomputed, not stored.  No reference counterpart. */
-int dft2_backward_batch(dft2_plan* plan, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
+DFT2_API int dft2_backward_batch(dft2_plan* plan, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
                         float sigma_scale, float* sigma_grad_dev, void* stream);
+
+/* ---- Batches: many clips, one graph launch ---------------------------------------- */
+
+typedef struct dft2_batch dft2_batch;
+
+typedef struct dft2_batch_options {
+    int autotune;      /* 1: time every valid tile shape and launch form on a sample of the
+                          batch and keep the fastest (tile_w/tile_h/persistent ignored) */
+    int tile_w;        /* blocks per workgroup tile along x (0 = the plan's default) */
+    int tile_h;        /* blocks per workgroup tile along y (0 = the plan's default) */
+    int persistent;    /* 1: persistent grid of resident workgroups over a work queue */
+    int sample_frames; /* autotune: output frames per job in the timing sample (0 = 2) */
+} dft2_batch_options;
+
+typedef struct dft2_batch_info {
+    int tile_w, tile_h;      /* chosen tile (blocks) */
+    int persistent;          /* chosen launch form */
+    int workgroups_per_cu;   /* resident fused workgroups per CU at this tile's LDS */
+    int compute_units;       /* CUs of the device */
+    int lds_bytes;           /* LDS per fused workgroup */
+    int n_chunks;            /* fused + finalize launch pairs in the graph */
+    long long work_items;    /* tiles x frames over all chunks */
+    long long scratch_bytes; /* tile-region scratch held by the batch */
+    int n_candidates;        /* candidates the autotuner timed (0 = not tuned) */
+    float tuned_ms;          /* autotuner's time of the chosen candidate on the sample */
+} dft2_batch_info;
+
+typedef struct dft2_batch_candidate {
+    int tile_w, tile_h;
+    int persistent;
+    int workgroups_per_cu;
+    int lds_bytes;
+    float ms;                /* average time on the autotuner's sample */
+} dft2_batch_candidate;
+
+/* Sets up `jobs` (device memory; the buffers must stay valid and at the same addresses
+ * for the batch's lifetime) for repeated runs.  options = NULL means autotune with the
+ * defaults.  With autotune the jobs' first sample_frames output frames are written
+ * while timing.  The batch holds its own scratch, job table and captured graph. */
+DFT2_API int dft2_batch_create(dft2_plan* plan, const dft2_job* jobs, int n_jobs, const dft2_batch_options* options,
+                      dft2_batch** out_batch);
+
+/* Runs the whole batch: one hipGraphLaunch on `stream` (NULL = the batch's own stream,
+ * and the call waits for completion).  Runs of one batch must not overlap: they share
+ * its scratch and work-queue counters. */
+DFT2_API int dft2_batch_launch(dft2_batch* batch, void* stream);
+
+/* Chosen configuration and sizes of the batch. */
+DFT2_API int dft2_batch_get_info(const dft2_batch* batch, dft2_batch_info* info);
+
+/* The autotuner's measurement for candidate `index` (0 .. n_candidates-1). */
+DFT2_API int dft2_batch_get_candidate(const dft2_batch* batch, int index, dft2_batch_candidate* out);
+
+/* Releases the batch (waits for its stream).  NULL is accepted. */
+DFT2_API int dft2_batch_destroy(dft2_batch* batch);
 
 /* FP64 real DFT of `data` viewed as `shape` (ndim 1..3), unnormalised, exponent -2*pi*i,
  * real along the last axis.  out receives prod(shape[:-1])*(shape[-1]/2+1) complex values
  * as interleaved (re, im) doubles; out_len is its capacity in doubles. */
-int dft2_rdft(const double* data, int data_len, const int* shape, int ndim, double* out, int out_len);
+DFT2_API int dft2_rdft(const double* data, int data_len, const int* shape, int ndim, double* out, int out_len);
 
 /* Version string of the library. */
-const char* dft2_version(void);
+DFT2_API const char* dft2_version(void);
 
 /* Message of the last error on the calling thread ("" if none). */
-const char* dft2_last_error(void);
+DFT2_API const char* dft2_last_error(void);
 
 #ifdef __cplusplus
 }

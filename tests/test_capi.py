@@ -4,6 +4,8 @@ validates arguments before touching a device, and its RDFT matches numpy."""
 import ctypes
 import os
 import re
+import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ HEADER = os.path.join(ROOT, "include", "dfttest2_mi355x.h")
 
 def declared_functions():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(dft2_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*DFT2_API\s+(?:int|const char\*)\s+(dft2_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -26,6 +28,17 @@ def test_library_exports_every_declared_symbol():
     assert set(names) == set(_lib.EXPORTED_SYMBOLS)
     for name in names:
         assert hasattr(lib, name), name
+
+
+def test_library_exports_only_the_c_abi():
+    nm = shutil.which("nm")
+    if nm is None:
+        pytest.skip("nm not available")
+    out = subprocess.run([nm, "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.split() and line.split()[-2] in "TW"}
+    ours = {n for n in names if "dft2" in n}
+    assert ours == set(declared_functions())
 
 
 def test_version_string():

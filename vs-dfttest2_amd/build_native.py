@@ -2,8 +2,9 @@
 
     python vs-dfttest2_amd/build_native.py
 
-Compiles csrc/dft2_kernels.hip (device + launch code) and csrc/dft2_capi.cpp (C ABI)
-with hipcc and links them into vs-dfttest2_amd/dfttest2_amd/_native/.
+Compiles csrc/dft2_kernels.hip (device + launch code), csrc/dft2_capi.cpp and
+csrc/dft2_batch.cpp (C ABI) with hipcc and links them into
+vs-dfttest2_amd/dfttest2_amd/_native/.
 """
 
 from __future__ import annotations
@@ -18,10 +19,13 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "dfttest2_amd", "_native")
 LIB_NAME = "libdfttest2_mi355x.so"
 ARCH = "gfx950"
+# Kernels and launch code; the C ABI (plans, batches).
+SOURCES = ("dft2_kernels.hip", "dft2_capi.cpp", "dft2_batch.cpp")
 
 # Packed FP32 instructions are kept out of every gfx950 code object.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
+# Only the C ABI (DFT2_API in include/dfttest2_mi355x.h) is exported from the library.
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden", "-Wall", "-Wno-unused-result"]
 
 
 def hipcc() -> str:
@@ -41,15 +45,15 @@ def build(verbose: bool = True) -> str:
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
     hc = hipcc()
-    k_obj = os.path.join(obj_dir, "dft2_kernels.o")
-    c_obj = os.path.join(obj_dir, "dft2_capi.o")
-    run([hc, "-c", "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS,
-         os.path.join(CSRC, "dft2_kernels.hip"), "-o", k_obj])
-    run([hc, "-c", "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS,
-         os.path.join(CSRC, "dft2_capi.cpp"), "-o", c_obj])
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
+        run([hc, "-c", "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS,
+             os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     out = os.path.join(OUT_DIR, LIB_NAME)
     tmp = out + ".tmp"
-    run([hc, "-shared", f"--offload-arch={ARCH}", k_obj, c_obj, "-o", tmp])
+    run([hc, "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, out)
     return out
 

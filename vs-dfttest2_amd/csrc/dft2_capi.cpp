@@ -8,22 +8,24 @@
 
 #include <algorithm>
 #include <cmath>
-#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <mutex>
 #include <string>
 #include <vector>
 
-#include "../../include/dfttest2_mi355x.h"
-#include "dft2_internal.h"
+#include "dft2_host.h"
 
 using namespace dft2;
+using namespace dft2::host;
+
+namespace dft2 {
+namespace host {
 
 namespace {
-
 thread_local std::string g_last_error;
+} // namespace
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -35,15 +37,14 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                                          \
-    do {                                                                                                       \
-        hipError_t e_ = (expr);                                                                                \
-        if (e_ != hipSuccess) {                                                                                \
-            return fail(DFT2_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, __LINE__); \
-        }                                                                                                      \
-    } while (0)
+void clear_error() {
+    g_last_error.clear();
+}
 
-// cpu_source/getframe_impl.cpp:11-20
+const char* last_error() {
+    return g_last_error.c_str();
+}
+
 int pad_size(int size, int bs, int step) {
     return size + ((size % bs) ? bs - size % bs : 0) + std::max(bs - step, step) * 2;
 }
@@ -52,52 +53,29 @@ int pad_num(int size, int bs, int step) {
     return (pad_size(size, bs, step) - bs) / step + 1;
 }
 
-constexpr int kRing = 8;
+bool tile_valid(const dft2_plan* p, int bw, int bh) {
+    // A pixel is covered by ceil(16 / S) consecutive blocks per axis; they fall into at
+    // most 2 tiles when a tile holds at least ceil(16 / S) - 1 blocks.
+    const int S = p->block_step;
+    const int need = std::max(1, (16 + S - 1) / S - 1);
+    return bw >= need && bh >= need && bw <= 64 && bh <= 64;
+}
 
-struct Slot {
-    hipStream_t stream = nullptr;
-    hipEvent_t done = nullptr;
-    float* scratch = nullptr;
-    size_t scratch_bytes = 0;
-    JobDev* d_jobs = nullptr;
-    size_t d_jobs_cap = 0;
-    JobDev* h_jobs[kRing] = {};
-    size_t h_jobs_cap[kRing] = {};
-    hipEvent_t h_jobs_done[kRing] = {};
-    int ring_pos = 0;
-    void* stage = nullptr; // host-memory path: T input planes + output plane
-    size_t stage_bytes = 0;
-    bool busy = false;
-};
+TileShape make_tile(const dft2_plan* p, int bw, int bh) {
+    TileShape ts;
+    ts.bw = bw;
+    ts.bh = bh;
+    ts.region_w = ((p->block_step * (bw - 1) + 16) + 3) / 4 * 4;
+    ts.region_h = p->block_step * (bh - 1) + 16;
+    ts.lds = fused_lds_bytes(p->radius, ts.region_w, ts.region_h);
+    return ts;
+}
 
-} // namespace
+} // namespace host
+} // namespace dft2
 
-struct dft2_plan {
-    int device = 0;
-    int radius = 1;
-    int t_len = 3;
-    int block_step = 4;
-    int filter_type = 0;
-    int zero_mean = 0;
-    int asym = 0;
-    int sample_code = kU8;
-    int precision = DFT2_PRECISION_FP32;
-    int bits = 8;
-    int bytes_per_sample = 1;
-    float out_scale = 1.0f;
-    int peak = 255;
-    float sigma2 = 0, pmin = 0, pmax = 0, wf0 = 1.0f;
-    int tile_bw = 16, tile_bh = 16;
-    int region_w = 0, region_h = 0;
-    size_t lds_bytes = 0;
-    size_t scratch_cap = size_t(1) << 30;
-    float* d_consts = nullptr;
-    std::vector<Slot> slots;
-    std::mutex mu;
-    std::condition_variable cv;
-};
-
-namespace {
+namespace dft2 {
+namespace host {
 
 // Builds the constant table (layout in dft2_internal.h) in FP64 and rounds to float.
 void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, float scale) {
@@ -250,14 +228,14 @@ int check_mirror(int size, int step, const char* what) {
 }
 
 // Fills the launch arguments common to every kernel of a plan.
-LaunchArgs make_args(const dft2_plan* p) {
+LaunchArgs make_args(const dft2_plan* p, const TileShape& ts) {
     LaunchArgs a{};
     a.consts = p->d_consts;
     a.block_step = p->block_step;
-    a.tile_bw = p->tile_bw;
-    a.tile_bh = p->tile_bh;
-    a.region_w = p->region_w;
-    a.region_h = p->region_h;
+    a.tile_bw = ts.bw;
+    a.tile_bh = ts.bh;
+    a.region_w = ts.region_w;
+    a.region_h = ts.region_h;
     a.inv_wf0 = 1.0f / p->wf0;
     a.sigma2 = p->sigma2;
     a.pmin = p->pmin;
@@ -270,6 +248,8 @@ LaunchArgs make_args(const dft2_plan* p) {
     a.sigma_dev = nullptr;
     a.sigma_scale = 1.0f;
     a.sigma_grad = nullptr;
+    a.queue = nullptr;
+    a.n_items = 0;
     return a;
 }
 
@@ -299,84 +279,95 @@ int upload_jobs(Slot* slot, const JobDev* src_table, size_t table_bytes, hipStre
     return DFT2_OK;
 }
 
-// Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
-int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
-             const float* sigma_dev = nullptr, float sigma_scale = 1.0f) {
+int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDev>& jobs_in, size_t scratch_cap,
+                ChunkPlan& out) {
     const int n_jobs = static_cast<int>(jobs_in.size());
-    const long long region = static_cast<long long>(p->region_w) * p->region_h;
+    const long long region = static_cast<long long>(ts.region_w) * ts.region_h;
     long long per_frame_items = 0;
     int max_frames = 0;
     for (const auto& j : jobs_in) {
         per_frame_items += static_cast<long long>(j.tiles_x) * j.tiles_y;
         max_frames = std::max(max_frames, j.f_count);
     }
+    out = ChunkPlan();
+    out.n_jobs = n_jobs;
     if (per_frame_items == 0 || max_frames == 0) {
         return DFT2_OK;
     }
-    long long frames_per_chunk = static_cast<long long>(p->scratch_cap / (per_frame_items * region * sizeof(float)));
+    long long frames_per_chunk = static_cast<long long>(scratch_cap / (per_frame_items * region * sizeof(float)));
+    // A launch has fewer than 2^31 work items (32-bit item arithmetic in the kernels).
+    frames_per_chunk = std::min<long long>(frames_per_chunk, (1LL << 31) / per_frame_items - 1);
     frames_per_chunk = std::max(1LL, std::min<long long>(frames_per_chunk, max_frames));
-    const int n_chunks = static_cast<int>((max_frames + frames_per_chunk - 1) / frames_per_chunk);
-    const size_t scratch_bytes = static_cast<size_t>(frames_per_chunk * per_frame_items * region) * sizeof(float);
-
-    int rc = ensure_device(reinterpret_cast<void**>(&slot->scratch), &slot->scratch_bytes, scratch_bytes);
-    if (rc) {
-        return rc;
+    if (per_frame_items >= (1LL << 30)) {
+        return fail(DFT2_ERR_ARG, "too many blocks in one frame of the batch");
     }
-    const size_t table_bytes = sizeof(JobDev) * static_cast<size_t>(n_jobs) * n_chunks;
-    // Device job table: reused across calls on this slot; stream order (or the wait on
-    // the slot's previous completion) keeps earlier kernels from seeing new data.
-    std::vector<JobDev> table(static_cast<size_t>(n_jobs) * n_chunks);
-    JobDev* h = table.data();
-    std::vector<long long> chunk_items(n_chunks), chunk_frames(n_chunks);
-    std::vector<int> chunk_maxpix(n_chunks);
+    const int n_chunks = static_cast<int>((max_frames + frames_per_chunk - 1) / frames_per_chunk);
+    out.scratch_bytes = static_cast<size_t>(frames_per_chunk * per_frame_items * region) * sizeof(float);
+    out.table.resize(static_cast<size_t>(n_jobs) * n_chunks);
+    out.chunks.resize(n_chunks);
     for (int c = 0; c < n_chunks; c++) {
-        long long items = 0, jf = 0;
-        int maxpix = 0;
+        Chunk& ch = out.chunks[c];
         for (int k = 0; k < n_jobs; k++) {
             JobDev j = jobs_in[k];
             const int f0 = static_cast<int>(c * frames_per_chunk);
             const int cnt = std::max(0, std::min<int>(static_cast<int>(frames_per_chunk), j.f_count - f0));
             j.f_begin = j.f_begin + f0;
             j.f_count = cnt;
-            j.item_begin = items;
-            j.frame_begin = jf;
-            items += static_cast<long long>(cnt) * j.tiles_x * j.tiles_y;
-            jf += cnt;
+            j.item_begin = ch.items;
+            j.frame_begin = ch.frames;
+            ch.items += static_cast<long long>(cnt) * j.tiles_x * j.tiles_y;
+            ch.frames += cnt;
             if (cnt > 0) {
-                maxpix = std::max(maxpix, j.width * j.height);
+                ch.max_pixels = std::max(ch.max_pixels, j.width * j.height);
             }
-            h[c * n_jobs + k] = j;
+            out.table[static_cast<size_t>(c) * n_jobs + k] = j;
         }
-        chunk_items[c] = items;
-        chunk_frames[c] = jf;
-        chunk_maxpix[c] = maxpix;
     }
-    rc = upload_jobs(slot, h, table_bytes, stream);
+    return DFT2_OK;
+}
+
+// Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
+int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
+             const float* sigma_dev = nullptr, float sigma_scale = 1.0f) {
+    const TileShape& ts = p->tile;
+    ChunkPlan cp;
+    int rc = plan_chunks(p, ts, jobs_in, p->scratch_cap, cp);
+    if (rc || cp.chunks.empty()) {
+        return rc;
+    }
+    rc = ensure_device(reinterpret_cast<void**>(&slot->scratch), &slot->scratch_bytes, cp.scratch_bytes);
     if (rc) {
         return rc;
     }
-
-    LaunchArgs a = make_args(p);
-    a.n_jobs = n_jobs;
+    // Device job table: reused across calls on this slot; stream order (or the wait on
+    // the slot's previous completion) keeps earlier kernels from seeing new data.
+    rc = upload_jobs(slot, cp.table.data(), sizeof(JobDev) * cp.table.size(), stream);
+    if (rc) {
+        return rc;
+    }
+    LaunchArgs a = make_args(p, ts);
+    a.n_jobs = cp.n_jobs;
     a.scratch = slot->scratch;
     if (sigma_dev) {
         a.sigma_dev = sigma_dev;
         a.sigma_scale = sigma_scale;
         a.asym = 1;
     }
-    for (int c = 0; c < n_chunks; c++) {
-        if (chunk_items[c] == 0) {
+    const FusedKey key = fused_key(p);
+    for (size_t c = 0; c < cp.chunks.size(); c++) {
+        const Chunk& ch = cp.chunks[c];
+        if (ch.items == 0) {
             continue;
         }
-        a.jobs = slot->d_jobs + static_cast<size_t>(c) * n_jobs;
-        HIP_TRY(launch_fused(a, p->radius, p->sample_code, p->precision, chunk_items[c], p->lds_bytes, stream));
-        HIP_TRY(launch_finalize(a, p->sample_code, chunk_frames[c], chunk_maxpix[c], stream));
+        a.jobs = slot->d_jobs + c * cp.n_jobs;
+        HIP_TRY(launch_fused(a, key, static_cast<unsigned>(ch.items), ts.lds, stream));
+        HIP_TRY(launch_finalize(a, p->sample_code, ch.frames, ch.max_pixels, stream));
     }
     return DFT2_OK;
 }
 
 // Fills geometry of a JobDev from the user's job description.
-int make_job(const dft2_plan* p, const dft2_job& uj, JobDev& j) {
+int make_job(const dft2_plan* p, const dft2_job& uj, const TileShape& ts, JobDev& j) {
     if (!uj.src || !uj.dst) {
         return fail(DFT2_ERR_ARG, "job src/dst must not be NULL");
     }
@@ -417,8 +408,8 @@ int make_job(const dft2_plan* p, const dft2_job& uj, JobDev& j) {
     j.nby = pad_num(uj.height, 16, p->block_step);
     j.ox = (j.pad_w - uj.width) / 2;
     j.oy = (j.pad_h - uj.height) / 2;
-    j.tiles_x = (j.nbx + p->tile_bw - 1) / p->tile_bw;
-    j.tiles_y = (j.nby + p->tile_bh - 1) / p->tile_bh;
+    j.tiles_x = (j.nbx + ts.bw - 1) / ts.bw;
+    j.tiles_y = (j.nby + ts.bh - 1) / ts.bh;
     j.use_fptr = 0;
     return DFT2_OK;
 }
@@ -428,12 +419,13 @@ int select_device(const dft2_plan* p) {
     return DFT2_OK;
 }
 
-} // namespace
+} // namespace host
+} // namespace dft2
 
 extern "C" {
 
 const char* dft2_last_error(void) {
-    return g_last_error.c_str();
+    return last_error();
 }
 
 const char* dft2_version(void) {
@@ -442,7 +434,7 @@ const char* dft2_version(void) {
 
 int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sample, int device_id, int num_streams,
                      int precision, dft2_plan** out_plan) {
-    g_last_error.clear();
+    clear_error();
     if (!prm || !out_plan) {
         return fail(DFT2_ERR_ARG, "params and out_plan must not be NULL");
     }
@@ -519,11 +511,7 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
     }
     // Tile of blocks per workgroup: accumulator region at most ~100 samples per side.
     const int tb = std::min(16, 84 / p->block_step + 1);
-    p->tile_bw = tb;
-    p->tile_bh = tb;
-    p->region_w = ((p->block_step * (tb - 1) + 16) + 3) / 4 * 4;
-    p->region_h = p->block_step * (tb - 1) + 16;
-    p->lds_bytes = fused_lds_bytes(p->radius, p->region_w, p->region_h);
+    p->tile = make_tile(p, tb, tb);
     if (const char* env = std::getenv("DFT2_SCRATCH_MB")) {
         long mb = std::atol(env);
         if (mb > 0) {
@@ -612,7 +600,7 @@ int dft2_process_batch(dft2_plan* p, const dft2_job* jobs, int n_jobs, void* str
 
 int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, const float* sigma_dev,
                              float sigma_scale, void* stream) {
-    g_last_error.clear();
+    clear_error();
     if (!p) {
         return fail(DFT2_ERR_ARG, "plan must not be NULL");
     }
@@ -621,7 +609,7 @@ int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, con
     }
     std::vector<JobDev> jd(n_jobs);
     for (int i = 0; i < n_jobs; i++) {
-        int rc = make_job(p, jobs[i], jd[i]);
+        int rc = make_job(p, jobs[i], p->tile, jd[i]);
         if (rc) {
             return rc;
         }
@@ -649,7 +637,7 @@ int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, con
 
 int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
                         float sigma_scale, float* sigma_grad_dev, void* stream) {
-    g_last_error.clear();
+    clear_error();
     if (!p || !sigma_dev || !sigma_grad_dev) {
         return fail(DFT2_ERR_ARG, "plan, sigma_dev and sigma_grad_dev must not be NULL");
     }
@@ -692,7 +680,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
         uj.n_frames = g.n_frames;
         uj.frame_begin = g.frame_begin;
         uj.frame_end = g.frame_end;
-        int rc = make_job(p, uj, jd[i]);
+        int rc = make_job(p, uj, p->tile, jd[i]);
         if (rc) {
             return rc;
         }
@@ -728,7 +716,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     if (rc) {
         return rc;
     }
-    LaunchArgs a = make_args(p);
+    LaunchArgs a = make_args(p, p->tile);
     a.jobs = slot->d_jobs;
     a.n_jobs = n_jobs;
     a.sigma_dev = sigma_dev;
@@ -749,7 +737,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
 
 int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long src_stride, void* dst,
                        long long dst_stride, int width, int height, int memory, void* stream) {
-    g_last_error.clear();
+    clear_error();
     if (!p || !src_frames || !dst) {
         return fail(DFT2_ERR_ARG, "plan, src_frames and dst must not be NULL");
     }
@@ -776,7 +764,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         return rc;
     }
     if (memory == 0) {
-        rc = make_job(p, uj, j);
+        rc = make_job(p, uj, p->tile, j);
         if (rc) {
             return rc;
         }
@@ -828,7 +816,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     uj.dst = stage + T * plane;
     uj.dst_stride = row;
     uj.dst_frame_pitch = 0;
-    rc = make_job(p, uj, j);
+    rc = make_job(p, uj, p->tile, j);
     if (rc) {
         return rc;
     }
@@ -843,7 +831,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
 }
 
 int dft2_rdft(const double* data, int data_len, const int* shape, int ndim, double* out, int out_len) {
-    g_last_error.clear();
+    clear_error();
     // Same transform as RDFT (cpu_source/source.cpp:269-340): naive separable DFT in FP64,
     // real along the last axis, then complex along the others.
     if (!data || !shape || !out || ndim < 1 || ndim > 3) {

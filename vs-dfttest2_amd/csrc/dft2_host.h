@@ -1,0 +1,127 @@
+// Host-side internals shared by the C ABI translation units (dft2_capi.cpp,
+// dft2_batch.cpp): the plan object, tile shapes, job validation and chunk planning.
+#pragma once
+
+#include <condition_variable>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/dfttest2_mi355x.h"
+#include "dft2_internal.h"
+
+namespace dft2 {
+namespace host {
+
+// Records the message for dft2_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...);
+void clear_error();
+const char* last_error();
+
+#define HIP_TRY(expr)                                                                                          \
+    do {                                                                                                       \
+        hipError_t e_ = (expr);                                                                                \
+        if (e_ != hipSuccess) {                                                                                \
+            return ::dft2::host::fail(DFT2_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),     \
+                                      __FILE__, __LINE__);                                                     \
+        }                                                                                                      \
+    } while (0)
+
+// cpu_source/getframe_impl.cpp:11-20
+int pad_size(int size, int bs, int step);
+int pad_num(int size, int bs, int step);
+
+constexpr int kRing = 8;
+
+// One concurrent call's stream and buffers (num_streams of them per plan).
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    float* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    JobDev* d_jobs = nullptr;
+    size_t d_jobs_cap = 0;
+    JobDev* h_jobs[kRing] = {};
+    size_t h_jobs_cap[kRing] = {};
+    hipEvent_t h_jobs_done[kRing] = {};
+    int ring_pos = 0;
+    void* stage = nullptr; // host-memory path: T input planes + output plane
+    size_t stage_bytes = 0;
+    bool busy = false;
+};
+
+// Tile of blocks owned by one workgroup of the fused kernel and its LDS footprint.
+struct TileShape {
+    int bw = 16, bh = 16;           // blocks per tile (x, y)
+    int region_w = 0, region_h = 0; // accumulator region of a full tile (samples)
+    size_t lds = 0;                 // dynamic LDS bytes of the fused kernel
+};
+
+// Work of one fused + finalize launch pair.
+struct Chunk {
+    long long items = 0;  // work items (tiles x frames)
+    long long frames = 0; // job-frames (finalize grid y)
+    int max_pixels = 0;   // largest plane among jobs with work (finalize grid x)
+};
+
+// Job table rows [c * n_jobs, (c+1) * n_jobs) describe chunk c.
+struct ChunkPlan {
+    int n_jobs = 0;
+    std::vector<JobDev> table;
+    std::vector<Chunk> chunks;
+    size_t scratch_bytes = 0;
+};
+
+} // namespace host
+} // namespace dft2
+
+struct dft2_plan {
+    int device = 0;
+    int radius = 1;
+    int t_len = 3;
+    int block_step = 4;
+    int filter_type = 0;
+    int zero_mean = 0;
+    int asym = 0;
+    int sample_code = dft2::kU8;
+    int precision = DFT2_PRECISION_FP32;
+    int bits = 8;
+    int bytes_per_sample = 1;
+    float out_scale = 1.0f;
+    int peak = 255;
+    float sigma2 = 0, pmin = 0, pmax = 0, wf0 = 1.0f;
+    dft2::host::TileShape tile; // default forward tile
+    size_t scratch_cap = size_t(1) << 30;
+    float* d_consts = nullptr;
+    std::vector<dft2::host::Slot> slots;
+    std::mutex mu;
+    std::condition_variable cv;
+};
+
+namespace dft2 {
+namespace host {
+
+inline FusedKey fused_key(const dft2_plan* p, int persistent = 0) {
+    return FusedKey{p->radius, p->sample_code, p->precision, p->filter_type, persistent};
+}
+
+// Forward tile of bw x bh blocks for plan p.  Valid when every pixel is covered by the
+// regions of at most 2 tiles per axis (the finalize kernel gathers at most 4).
+TileShape make_tile(const dft2_plan* p, int bw, int bh);
+bool tile_valid(const dft2_plan* p, int bw, int bh);
+
+// Launch arguments common to every kernel of a plan, for tile `ts`.
+LaunchArgs make_args(const dft2_plan* p, const TileShape& ts);
+
+// Validates one user job and fills its geometry (tiles counted for `ts`).
+int make_job(const dft2_plan* p, const dft2_job& uj, const TileShape& ts, JobDev& j);
+
+// Splits jobs into frame chunks whose scratch fits `scratch_cap` bytes.
+int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDev>& jobs, size_t scratch_cap,
+                ChunkPlan& out);
+
+int ensure_device(void** ptr, size_t* cap, size_t bytes);
+int select_device(const dft2_plan* p);
+
+} // namespace host
+} // namespace dft2

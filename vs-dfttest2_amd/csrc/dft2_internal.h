@@ -75,13 +75,33 @@ struct LaunchArgs {
     const float* sigma_dev;
     float sigma_scale;
     float* sigma_grad;          // backward: d loss / d sigma_dev, accumulated with +=
+    // Persistent launch: work items [0, n_items) are taken from the counter *queue
+    // (zeroed before the launch); nullptr = one work item per workgroup.  A launch has
+    // fewer than 2^31 work items.
+    unsigned* queue;
+    int n_items;
+};
+
+// Selects the fused kernel instantiation.
+struct FusedKey {
+    int radius;
+    int sample_code;
+    int precision;
+    int filter_type;
+    int persistent; // 1: persistent grid over the work queue (LaunchArgs::queue)
 };
 
 // Sample type codes.
 enum SampleCode { kU8 = 0, kU16 = 1, kF32 = 2 };
 
-hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, int precision, long long n_items,
-                        size_t lds_bytes, hipStream_t stream);
+// Fused kernel over `grid` workgroups: one work item each, or a persistent grid over the
+// work queue when a.queue is set.
+hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, size_t lds_bytes,
+                        hipStream_t stream);
+// Workgroups of the fused kernel resident per CU with `lds_bytes` of LDS (0 on error).
+int fused_workgroups_per_cu(const FusedKey& k, size_t lds_bytes);
+// Zeroes n work-queue counters with a kernel (safe inside graph capture).
+hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream);
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,
                            hipStream_t stream);
 hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds_bytes, hipStream_t stream);

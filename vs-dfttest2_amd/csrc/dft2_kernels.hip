@@ -184,6 +184,24 @@ __device__ __forceinline__ int sigma_index(int3 p) {
     return (p.x * 16 + p.y) * 9 + p.z;
 }
 
+// Index of the last job whose first work item (or first job-frame) is <= v: the job
+// table is sorted by both.  Jobs without work in a chunk share their start with the
+// next job and are skipped by taking the last match.
+template <bool kByFrame>
+__device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, long long v) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const long long start = kByFrame ? jobs[mid].frame_begin : jobs[mid].item_begin;
+        if (start <= v) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
 // Gain table into LDS: the plan's table, with sigma taken from a.sigma_dev when given.
 template <int R>
 __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int tid, int n_threads) {
@@ -201,7 +219,7 @@ __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int t
     }
 }
 
-template <int R, typename Tin, int FTM, int PREC>
+template <int R, typename Tin, int FTM, int PREC, bool PERSIST>
 __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
     constexpr int kWbuf = (T * 256 > 320) ? T * 256 : 320;
@@ -213,36 +231,26 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     const int l16 = lane & 15;
     const int lg = lane >> 4;
 
-    // ---- work item -> (job, frame, tile) ----
-    const long long item = blockIdx.x;
-    int jb = 0;
-    while (jb + 1 < a.n_jobs && a.jobs[jb + 1].item_begin <= item) {
-        ++jb;
+    // Work items: one per workgroup, or (PERSIST) taken in turn from the work-queue
+    // counter a.queue by a persistent grid until a.n_items are done.  The two forms are
+    // separate instantiations: the loop costs registers the one-item form does not need.
+    // Work items of one launch number fewer than 2^31 (the host splits larger work), so
+    // item arithmetic stays 32-bit.
+    __shared__ int s_item;
+    int item = blockIdx.x;
+    if constexpr (PERSIST) {
+        if (tid == 0) {
+            s_item = static_cast<int>(atomicAdd(a.queue, 1u));
+        }
+        __syncthreads();
+        item = __builtin_amdgcn_readfirstlane(s_item);
     }
-    const JobDev& job = a.jobs[jb];
-    const int tiles = job.tiles_x * job.tiles_y;
-    const long long local = item - job.item_begin;
-    const int f_local = static_cast<int>(local / tiles);
-    const int tile = static_cast<int>(local - static_cast<long long>(f_local) * tiles);
-    const int ty = tile / job.tiles_x;
-    const int tx = tile - ty * job.tiles_x;
-    const int n = job.f_begin + f_local;
-    const int S = a.block_step;
-    const int by0 = ty * a.tile_bh;
-    const int bx0 = tx * a.tile_bw;
-    const int nby = min(a.tile_bh, job.nby - by0);
-    const int nbx = min(a.tile_bw, job.nbx - bx0);
-    const int nblk = nby * nbx;
-    const int rw = a.region_w;
-    const int rows_used = S * (nby - 1) + 16;
 
+    const int S = a.block_step;
+    const int rw = a.region_w;
     float* acc = smem;
     float4* tab = reinterpret_cast<float4*>(smem + a.region_h * rw);
     float* wbuf = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64) + wave * kWbuf;
-
-    for (int i = tid; i < rows_used * rw; i += kThreads) {
-        acc[i] = 0.0f;
-    }
     fill_tab<R>(tab, a, tid, kThreads);
 
     // ---- per-lane constant operands ----
@@ -275,6 +283,37 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         }
     }
 
+    // Middle-stage ownership: lane = 8*p + q owns y pair p (rows m = p, p+8 of the real
+    // basis) and x pair q (columns n = q, q+8).  Pair 0 is (cos 0, cos 8) on both axes,
+    // two purely real-axis coefficients; pairs 1..7 are (cos k, sin k).
+    const int mp = lane >> 3;
+    const int mq = lane & 7;
+    const bool ey = mp != 0;
+    const bool ex = mq != 0;
+    const int wq = l16 & 7;
+    const int wc = l16 >> 3;
+
+    while (item < a.n_items) {
+    // ---- work item -> (job, frame, tile) ----
+    const JobDev& job = a.jobs[find_job<false>(a.jobs, a.n_jobs, item)];
+    const int tiles = job.tiles_x * job.tiles_y;
+    const int local = item - static_cast<int>(job.item_begin);
+    const int f_local = local / tiles;
+    const int tile = local - f_local * tiles;
+    const int ty = tile / job.tiles_x;
+    const int tx = tile - ty * job.tiles_x;
+    const int n = job.f_begin + f_local;
+    const int by0 = ty * a.tile_bh;
+    const int bx0 = tx * a.tile_bw;
+    const int nby = min(a.tile_bh, job.nby - by0);
+    const int nbx = min(a.tile_bw, job.nbx - bx0);
+    const int nblk = nby * nbx;
+    const int rows_used = S * (nby - 1) + 16;
+
+    for (int i = tid; i < rows_used * rw; i += kThreads) {
+        acc[i] = 0.0f;
+    }
+
     // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
     gptr_t<Tin> frame[T];
 #pragma unroll
@@ -303,16 +342,6 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
             }
         }
     };
-
-    // Middle-stage ownership: lane = 8*p + q owns y pair p (rows m = p, p+8 of the real
-    // basis) and x pair q (columns n = q, q+8).  Pair 0 is (cos 0, cos 8) on both axes,
-    // two purely real-axis coefficients; pairs 1..7 are (cos k, sin k).
-    const int mp = lane >> 3;
-    const int mq = lane & 7;
-    const bool ey = mp != 0;
-    const bool ex = mq != 0;
-    const int wq = l16 & 7;
-    const int wc = l16 >> 3;
 
     // The next block's samples are fetched one iteration ahead, so their load latency
     // overlaps the current block's transforms.
@@ -464,9 +493,19 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     }
     __syncthreads();
 
-    float* out = a.scratch + item * static_cast<long long>(a.region_h * rw);
+    float* out = a.scratch + static_cast<long long>(item) * (a.region_h * rw);
     for (int i = tid; i < rows_used * rw; i += kThreads) {
         out[i] = acc[i];
+    }
+    if constexpr (!PERSIST) {
+        break;
+    }
+    __syncthreads(); // the accumulator is read out and s_item consumed before reuse
+    if (tid == 0) {
+        s_item = static_cast<int>(atomicAdd(a.queue, 1u));
+    }
+    __syncthreads();
+    item = __builtin_amdgcn_readfirstlane(s_item);
     }
 }
 
@@ -533,11 +572,7 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     const int lg = lane >> 4;
 
     const long long item = blockIdx.x;
-    int jb = 0;
-    while (jb + 1 < a.n_jobs && a.jobs[jb + 1].item_begin <= item) {
-        ++jb;
-    }
-    const JobDev& job = a.jobs[jb];
+    const JobDev& job = a.jobs[find_job<false>(a.jobs, a.n_jobs, item)];
     const int tiles = job.tiles_x * job.tiles_y;
     const long long local = item - job.item_begin;
     const int f_local = static_cast<int>(local / tiles);
@@ -863,11 +898,7 @@ __device__ __forceinline__ void store_sample(Tout* p, float v, const LaunchArgs&
 template <typename Tout>
 __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
     const long long jf = blockIdx.y;
-    int jb = 0;
-    while (jb + 1 < a.n_jobs && a.jobs[jb + 1].frame_begin <= jf) {
-        ++jb;
-    }
-    const JobDev& job = a.jobs[jb];
+    const JobDev& job = a.jobs[find_job<true>(a.jobs, a.n_jobs, jf)];
     const int f_local = static_cast<int>(jf - job.frame_begin);
     const int pix = blockIdx.x * 256 + threadIdx.x;
     if (pix >= job.width * job.height) {
@@ -913,50 +944,59 @@ __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
     store_sample(dst, sum, a);
 }
 
-template <int R, typename Tin, int FTM, int PREC>
-hipError_t launch_one(const LaunchArgs& a, long long n_items, size_t lds, hipStream_t stream) {
-    auto kern = fused_kernel<R, Tin, FTM, PREC>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
-        if (e != hipSuccess) {
-            return e;
-        }
+template <int R, typename Tin, int PREC, bool PERSIST>
+const void* fused_ptr_ft(int filter_type) {
+    // Gains: 0 Wiener, 1 hard threshold, 2 the generic form for types 2..6.
+    if (filter_type == 0) {
+        return reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST>);
     }
-    hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(n_items)), dim3(kThreads), lds, stream, a);
-    return hipGetLastError();
+    if (filter_type == 1) {
+        return reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST>);
+    }
+    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, PERSIST>);
 }
 
-template <int R, typename Tin, int PREC>
-hipError_t launch_ft(const LaunchArgs& a, long long n_items, size_t lds, hipStream_t stream) {
-    if (a.filter_type == 0) {
-        return launch_one<R, Tin, 0, PREC>(a, n_items, lds, stream);
-    }
-    if (a.filter_type == 1) {
-        return launch_one<R, Tin, 1, PREC>(a, n_items, lds, stream);
-    }
-    return launch_one<R, Tin, 2, PREC>(a, n_items, lds, stream);
+template <int R, typename Tin, bool PERSIST>
+const void* fused_ptr_prec(int precision, int filter_type) {
+    return precision == 1 ? fused_ptr_ft<R, Tin, 1, PERSIST>(filter_type)
+                          : fused_ptr_ft<R, Tin, 0, PERSIST>(filter_type);
 }
 
-template <int R, typename Tin>
-hipError_t launch_prec(const LaunchArgs& a, int precision, long long n_items, size_t lds, hipStream_t stream) {
-    if (precision == 1) {
-        return launch_ft<R, Tin, 1>(a, n_items, lds, stream);
-    }
-    return launch_ft<R, Tin, 0>(a, n_items, lds, stream);
-}
-
-template <int R>
-hipError_t launch_type(const LaunchArgs& a, int sample_code, int precision, long long n_items, size_t lds,
-                       hipStream_t stream) {
+template <int R, bool PERSIST>
+const void* fused_ptr_type(int sample_code, int precision, int filter_type) {
     switch (sample_code) {
     case kU8:
-        return launch_prec<R, uint8_t>(a, precision, n_items, lds, stream);
+        return fused_ptr_prec<R, uint8_t, PERSIST>(precision, filter_type);
     case kU16:
-        return launch_prec<R, uint16_t>(a, precision, n_items, lds, stream);
+        return fused_ptr_prec<R, uint16_t, PERSIST>(precision, filter_type);
     default:
-        return launch_prec<R, float>(a, precision, n_items, lds, stream);
+        return fused_ptr_prec<R, float, PERSIST>(precision, filter_type);
     }
+}
+
+template <bool PERSIST>
+const void* fused_ptr_radius(const FusedKey& k) {
+    switch (k.radius) {
+    case 0:
+        return fused_ptr_type<0, PERSIST>(k.sample_code, k.precision, k.filter_type);
+    case 1:
+        return fused_ptr_type<1, PERSIST>(k.sample_code, k.precision, k.filter_type);
+    case 2:
+        return fused_ptr_type<2, PERSIST>(k.sample_code, k.precision, k.filter_type);
+    default:
+        return fused_ptr_type<3, PERSIST>(k.sample_code, k.precision, k.filter_type);
+    }
+}
+
+const void* fused_ptr(const FusedKey& k) {
+    return k.persistent ? fused_ptr_radius<true>(k) : fused_ptr_radius<false>(k);
+}
+
+hipError_t allow_lds(const void* kern, size_t lds) {
+    if (lds > 65536) {
+        return hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    }
+    return hipSuccess;
 }
 
 } // namespace
@@ -998,18 +1038,46 @@ size_t fused_lds_bytes(int radius, int region_w, int region_h) {
     return sizeof(float) * (static_cast<size_t>(region_w) * region_h + (radius + 1) * 4 * 64 * 4 + kWaves * wbuf);
 }
 
-hipError_t launch_fused(const LaunchArgs& a, int radius, int sample_code, int precision, long long n_items,
-                        size_t lds_bytes, hipStream_t stream) {
-    switch (radius) {
-    case 0:
-        return launch_type<0>(a, sample_code, precision, n_items, lds_bytes, stream);
-    case 1:
-        return launch_type<1>(a, sample_code, precision, n_items, lds_bytes, stream);
-    case 2:
-        return launch_type<2>(a, sample_code, precision, n_items, lds_bytes, stream);
-    default:
-        return launch_type<3>(a, sample_code, precision, n_items, lds_bytes, stream);
+hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, size_t lds_bytes,
+                        hipStream_t stream) {
+    if (k.persistent != (a.queue != nullptr)) {
+        return hipErrorInvalidValue;
     }
+    const void* kern = fused_ptr(k);
+    hipError_t e = allow_lds(kern, lds_bytes);
+    if (e != hipSuccess) {
+        return e;
+    }
+    LaunchArgs args = a;
+    if (!k.persistent) {
+        args.n_items = static_cast<int>(grid);
+    }
+    void* params[] = {&args};
+    return hipLaunchKernel(kern, dim3(grid), dim3(kThreads), params, lds_bytes, stream);
+}
+
+int fused_workgroups_per_cu(const FusedKey& k, size_t lds_bytes) {
+    const void* kern = fused_ptr(k);
+    if (allow_lds(kern, lds_bytes) != hipSuccess) {
+        return 0;
+    }
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kern, kThreads, lds_bytes) != hipSuccess) {
+        return 0;
+    }
+    return n;
+}
+
+__global__ void queue_reset_kernel(unsigned* q, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        q[i] = 0;
+    }
+}
+
+hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream) {
+    hipLaunchKernelGGL(queue_reset_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, queue, n);
+    return hipGetLastError();
 }
 
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,

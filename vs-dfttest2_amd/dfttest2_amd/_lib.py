@@ -26,6 +26,11 @@ EXPORTED_SYMBOLS = (
     "dft2_process_batch",
     "dft2_process_batch_sigma",
     "dft2_backward_batch",
+    "dft2_batch_create",
+    "dft2_batch_launch",
+    "dft2_batch_get_info",
+    "dft2_batch_get_candidate",
+    "dft2_batch_destroy",
     "dft2_rdft",
     "dft2_version",
     "dft2_last_error",
@@ -88,6 +93,43 @@ class Dft2GradJob(ctypes.Structure):
     ]
 
 
+class Dft2BatchOptions(ctypes.Structure):
+    _fields_ = [
+        ("autotune", ctypes.c_int),
+        ("tile_w", ctypes.c_int),
+        ("tile_h", ctypes.c_int),
+        ("persistent", ctypes.c_int),
+        ("sample_frames", ctypes.c_int),
+    ]
+
+
+class Dft2BatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("tile_w", ctypes.c_int),
+        ("tile_h", ctypes.c_int),
+        ("persistent", ctypes.c_int),
+        ("workgroups_per_cu", ctypes.c_int),
+        ("compute_units", ctypes.c_int),
+        ("lds_bytes", ctypes.c_int),
+        ("n_chunks", ctypes.c_int),
+        ("work_items", ctypes.c_longlong),
+        ("scratch_bytes", ctypes.c_longlong),
+        ("n_candidates", ctypes.c_int),
+        ("tuned_ms", ctypes.c_float),
+    ]
+
+
+class Dft2BatchCandidate(ctypes.Structure):
+    _fields_ = [
+        ("tile_w", ctypes.c_int),
+        ("tile_h", ctypes.c_int),
+        ("persistent", ctypes.c_int),
+        ("workgroups_per_cu", ctypes.c_int),
+        ("lds_bytes", ctypes.c_int),
+        ("ms", ctypes.c_float),
+    ]
+
+
 class Dft2Error(RuntimeError):
     pass
 
@@ -126,6 +168,19 @@ def load():
         ctypes.c_void_p, ctypes.c_void_p,
     ]
     lib.dft2_backward_batch.restype = ctypes.c_int
+    lib.dft2_batch_create.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.POINTER(Dft2BatchOptions),
+        ctypes.POINTER(ctypes.c_void_p),
+    ]
+    lib.dft2_batch_create.restype = ctypes.c_int
+    lib.dft2_batch_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    lib.dft2_batch_launch.restype = ctypes.c_int
+    lib.dft2_batch_get_info.argtypes = [ctypes.c_void_p, ctypes.POINTER(Dft2BatchInfo)]
+    lib.dft2_batch_get_info.restype = ctypes.c_int
+    lib.dft2_batch_get_candidate.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(Dft2BatchCandidate)]
+    lib.dft2_batch_get_candidate.restype = ctypes.c_int
+    lib.dft2_batch_destroy.argtypes = [ctypes.c_void_p]
+    lib.dft2_batch_destroy.restype = ctypes.c_int
     lib.dft2_rdft.argtypes = [_dbl_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _dbl_p, ctypes.c_int]
     lib.dft2_rdft.restype = ctypes.c_int
     lib.dft2_version.argtypes = []
@@ -225,3 +280,50 @@ class Plan:
         ptrs = (ctypes.c_void_p * len(src_ptrs))(*src_ptrs)
         _check(load().dft2_process_frame(self._h, ptrs, src_stride, ctypes.c_void_p(dst), dst_stride, width, height,
                                          memory, ctypes.c_void_p(stream or 0)))
+
+
+class Batch:
+    """A fixed set of jobs on one plan, run as one captured graph launch (dft2_batch_*).
+
+    autotune=True times every valid tile shape and launch form on the first
+    `sample_frames` output frames of every job and keeps the fastest; otherwise tile_w /
+    tile_h (0 = the plan's default) and persistent are used as given.  The job buffers
+    must stay allocated, at the same addresses, while the batch exists."""
+
+    def __init__(self, plan: Plan, jobs: List[Dft2Job], autotune: bool = True, tile_w: int = 0, tile_h: int = 0,
+                 persistent: bool = False, sample_frames: int = 0):
+        lib = load()
+        self.plan = plan  # keeps the plan alive
+        arr = (Dft2Job * len(jobs))(*jobs)
+        opt = Dft2BatchOptions(autotune=int(autotune), tile_w=tile_w, tile_h=tile_h, persistent=int(persistent),
+                               sample_frames=sample_frames)
+        handle = ctypes.c_void_p()
+        _check(lib.dft2_batch_create(plan._h, arr, len(jobs), ctypes.byref(opt), ctypes.byref(handle)))
+        self._h = handle
+
+    def launch(self, stream: Optional[int] = None):
+        _check(load().dft2_batch_launch(self._h, ctypes.c_void_p(stream or 0)))
+
+    def info(self) -> dict:
+        out = Dft2BatchInfo()
+        _check(load().dft2_batch_get_info(self._h, ctypes.byref(out)))
+        return {name: getattr(out, name) for name, _ in Dft2BatchInfo._fields_}
+
+    def candidates(self) -> list:
+        res = []
+        for i in range(self.info()["n_candidates"]):
+            c = Dft2BatchCandidate()
+            _check(load().dft2_batch_get_candidate(self._h, i, ctypes.byref(c)))
+            res.append({name: getattr(c, name) for name, _ in Dft2BatchCandidate._fields_})
+        return res
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().dft2_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
