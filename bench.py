@@ -26,6 +26,11 @@ for _p in (ROOT, os.path.join(ROOT, "vs-dfttest2_amd")):
 METRIC = "Mpx/s at 1080p & 4K, plain FP32 vs split-BF16 MFMA at ≤1 LSB; fwd+bwd Mpx/s"
 WIDTH, HEIGHT, FRAMES = 1920, 1080, 240
 PEAK_FP32_MFMA_TFLOPS = 157.3
+# HBM bytes per fused launch on the headline workload (one launch per step), from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a plain
+# copy kernel of known size (profiles/fused_kernel_1080p_yuv420_u8_fp32.md); None until
+# measured for the current kernel.
+FUSED_TRAFFIC_BYTES = None
 
 
 def flop_per_block(t_len: int) -> int:
@@ -81,9 +86,10 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "split_bf16"], default="fp32")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
-    ap.add_argument("--workload", choices=["fp32_1080p", "precision4k", "diff"], default="fp32_1080p",
+    ap.add_argument("--workload", choices=["fp32_1080p", "precision4k", "batch", "diff"], default="fp32_1080p",
                     help="fp32_1080p: the headline line; precision4k: 4K 10-bit FP32 vs split-BF16 with "
-                         "errors vs FP64; diff: forward vs forward+backward of the differentiable filter")
+                         "errors vs FP64; batch: 64 uneven float clips as one autotuned graph launch; "
+                         "diff: forward vs forward+backward of the differentiable filter")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path has no multi-GPU form here (use --gpus 1)",
@@ -106,6 +112,8 @@ def main():
         return bench_precision4k(args)
     if args.workload == "diff":
         return bench_diff(args)
+    if args.workload == "batch":
+        return bench_batch(args)
 
     # Whole clip in one fused launch: scratch for 240 frames of tile regions.
     os.environ.setdefault("DFT2_SCRATCH_MB", "8192")
@@ -131,23 +139,32 @@ def main():
     torch.cuda.synchronize()
     print(f"[bench] warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
 
+    # Step time from events on the launch stream; the library additionally records HIP
+    # events around each fused / finalize launch (same stream) for the kernel roofline.
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    plan.set_timing(True)
     ev0.record(stream)
     for _ in range(args.steps):
         plan.process_batch(jobs, handle)
     ev1.record(stream)
     torch.cuda.synchronize()
+    kt = plan.timing()
+    plan.set_timing(False)
     ms_per_step = ev0.elapsed_time(ev1) / args.steps
-    print(f"[bench] {args.steps} steps, {ms_per_step:.3f} ms/step", file=sys.stderr, flush=True)
+    fused_ms = kt["fused_ms"] / max(1, kt["fused_launches"])
+    print(f"[bench] {args.steps} steps, {ms_per_step:.3f} ms/step; fused kernel {fused_ms:.3f} ms/launch over "
+          f"{kt['fused_launches']} launches, finalize {kt['finalize_ms'] / max(1, kt['finalize_launches']):.3f} ms",
+          file=sys.stderr, flush=True)
 
     luma_mpx = WIDTH * HEIGHT * FRAMES / 1e6
     value = luma_mpx / (ms_per_step / 1e3)
     t_len = params.t_len
     blocks = (blocks_per_plane(WIDTH, HEIGHT) + 2 * blocks_per_plane(WIDTH // 2, HEIGHT // 2)) * FRAMES
-    flops = blocks * flop_per_block(t_len)
-    achieved = flops / (ms_per_step / 1e3) / 1e12
+    launches_per_step = kt["fused_launches"] / args.steps
+    flops_per_launch = blocks * flop_per_block(t_len) / launches_per_step
+    achieved = flops_per_launch / (fused_ms / 1e3) / 1e12
 
     result = {
         "metric": METRIC,
@@ -175,7 +192,10 @@ def main():
             "peak": PEAK_FP32_MFMA_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_MFMA_TFLOPS, 4),
-            "traffic": None,
+            "traffic": FUSED_TRAFFIC_BYTES if args.precision == "fp32" else None,
+            "kernel": "fused_kernel",
+            "kernel_ms": round(fused_ms, 4),
+            "launches_per_step": launches_per_step,
         },
     }
     if not args.no_cpu_baseline:
@@ -250,6 +270,73 @@ def bench_precision4k(args):
     res["speedup_split_bf16"] = round(res["fp32"]["ms_per_step"] / res["split_bf16"]["ms_per_step"], 3)
     print(json.dumps({"metric": METRIC, "workload": "precision4k: YUV420P10 3840x2160 + 2x1920x1080, 120 frames, "
                       "ftype 1, tbsize 5, slocation 0.0:4 0.5:8 1.0:16", "unit": "fps", "results": res}), flush=True)
+
+
+def batch_clips(n_clips: int = 64, seed: int = 3):
+    """Uneven clips: width ~ U{320..1920}, height ~ U{240..1080} (even), 8..32 frames,
+    noise sd ~ U[0.03, 0.08], clip i from generator seed 1000 + i."""
+    import numpy as np
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for i in range(n_clips):
+        w = 2 * int(rng.integers(160, 961))
+        h = 2 * int(rng.integers(120, 541))
+        n = int(rng.integers(8, 33))
+        sd = float(rng.uniform(0.03, 0.08))
+        out.append((n, h, w, sd, 1000 + i))
+    return out
+
+
+def bench_batch(args):
+    """64 float32 single-plane clips of uneven size and length, ftype 0, sbsize 16,
+    sosize 12, tbsize 3: one autotuned batch (dft2_batch_*) run as one graph launch per
+    step, beside the same jobs through dft2_process_batch with the plan's default tile."""
+    os.environ.setdefault("DFT2_SCRATCH_MB", "8192")
+    import torch
+
+    from dfttest2_amd import _lib, synth
+    from dfttest2_amd.dfttest2 import Backend, get_plan, plane_job
+    from dfttest2_amd.params import dfttest_params
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    params = dfttest_params()
+    clips = batch_clips()
+    print(f"[bench] generating {len(clips)} clips ({sum(c[0] for c in clips)} frames)", file=sys.stderr, flush=True)
+    xs = [synth.float_plane(n, h, w, seed, sd, device=dev) for n, h, w, sd, seed in clips]
+    outs = [torch.empty_like(x) for x in xs]
+    jobs = [plane_job(x, o) for x, o in zip(xs, outs)]
+    pixels = sum(n * h * w for n, h, w, _, _ in clips)
+    plan = get_plan(params, "float", 32, Backend.MI355X())
+    stream = torch.cuda.current_stream(dev)
+
+    t0 = time.perf_counter()
+    batch = _lib.Batch(plan, jobs, autotune=True)
+    tune_s = time.perf_counter() - t0
+    info = batch.info()
+    print(f"[bench] autotune {tune_s:.2f} s: tile {info['tile_w']}x{info['tile_h']} persistent {info['persistent']}",
+          file=sys.stderr, flush=True)
+    ms_graph = _time_steps(lambda: batch.launch(stream.cuda_stream), args.steps, args.warmup, stream)
+    ms_plain = _time_steps(lambda: plan.process_batch(jobs, stream.cuda_stream), args.steps, args.warmup, stream)
+    blocks = sum(n * blocks_per_plane(w, h) for n, h, w, _, _ in clips)
+    achieved = blocks * flop_per_block(params.t_len) / (ms_graph / 1e3) / 1e12
+    cands = sorted(batch.candidates(), key=lambda c: c["ms"])
+    print(json.dumps({
+        "metric": METRIC, "workload": "batch: 64 float32 clips, W U{320..1920}, H U{240..1080}, 8-32 frames, "
+        "ftype 0 sbsize 16 sosize 12 tbsize 3, one autotuned graph launch per step",
+        "unit": "Mpx/s", "value": round(pixels / (ms_graph / 1e3) / 1e6, 1), "ms_per_step": round(ms_graph, 3),
+        "pixels_per_step": pixels, "frames_per_step": sum(c[0] for c in clips),
+        "plain_process_batch": {"ms_per_step": round(ms_plain, 3),
+                                "mpx_s": round(pixels / (ms_plain / 1e3) / 1e6, 1)},
+        "speedup_vs_plain": round(ms_plain / ms_graph, 3),
+        "tflops_step": round(achieved, 2),
+        "autotune": {"seconds": round(tune_s, 2), "chosen": {k: info[k] for k in ("tile_w", "tile_h", "persistent",
+                     "workgroups_per_cu", "lds_bytes", "n_chunks", "work_items")},
+                     "candidates": [[c["tile_w"], c["tile_h"], c["persistent"], c["workgroups_per_cu"], round(c["ms"], 3)]
+                                    for c in cands]},
+    }), flush=True)
+    batch.close()
 
 
 def bench_diff(args):

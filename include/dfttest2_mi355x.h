@@ -147,6 +147,24 @@ DFT2_API int dft2_process_batch_sigma(dft2_plan* plan, const dft2_job* jobs, int
 DFT2_API int dft2_backward_batch(dft2_plan* plan, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
                         float sigma_scale, float* sigma_grad_dev, void* stream);
 
+/* ---- Kernel timing ----------------------------------------------------------------- */
+
+typedef struct dft2_kernel_timing {
+    long long fused_launches;    /* fused kernel launches recorded */
+    double fused_ms;             /* their summed duration */
+    long long finalize_launches; /* finalize kernel launches recorded */
+    double finalize_ms;          /* their summed duration */
+} dft2_kernel_timing;
+
+/* enable = 1: from now on the plan's process calls record HIP events around every fused
+ * and finalize launch, on the stream the launch uses (two extra event records per
+ * launch); the totals start from zero.  enable = 0 stops recording.  No reference
+ * counterpart: instrumentation for throughput and roofline measurement. */
+DFT2_API int dft2_plan_set_timing(dft2_plan* plan, int enable);
+
+/* Waits for the recorded events and returns the totals since timing was enabled. */
+DFT2_API int dft2_plan_get_timing(dft2_plan* plan, dft2_kernel_timing* out);
+
 /* ---- Batches: many clips, one graph launch ---------------------------------------- */
 
 typedef struct dft2_batch dft2_batch;

@@ -139,8 +139,10 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
                     const int kt = pos[k][0], ky2 = pos[k][1], kx = pos[k][2];
                     const bool edge = kx == 0 || kx == 8;
                     float* e4 = &c[kOffTAB + (((g * 4 + 2 * e + k) * 64) + lane) * 4];
-                    e4[0] = sigma_at(kt, ky2, kx);
-                    e4[1] = edge ? sigma_at((T - kt) % T, (16 - ky2) % 16, kx) : e4[0];
+                    // Wiener plans keep sigma + 1e-15, the form the kernel's gain takes.
+                    const float bias = p->filter_type == 0 ? 1e-15f : 0.0f;
+                    e4[0] = sigma_at(kt, ky2, kx) + bias;
+                    e4[1] = (edge ? sigma_at((T - kt) % T, (16 - ky2) % 16, kx) : sigma_at(kt, ky2, kx)) + bias;
                     e4[2] = wf_at(kt, ky2, kx, 0);
                     e4[3] = wf_at(kt, ky2, kx, 1);
                 }
@@ -326,6 +328,25 @@ int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDe
     return DFT2_OK;
 }
 
+// With timing enabled, hands out the next three events of the plan's pool (creating
+// them on first use); otherwise leaves ev[] null.
+int timing_events(dft2_plan* p, hipEvent_t ev[3]) {
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (!p->timing) {
+        return DFT2_OK;
+    }
+    while (p->timing_events.size() < p->timing_used + 3) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e));
+        p->timing_events.push_back(e);
+    }
+    for (int i = 0; i < 3; i++) {
+        ev[i] = p->timing_events[p->timing_used + i];
+    }
+    p->timing_used += 3;
+    return DFT2_OK;
+}
+
 // Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
 int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
              const float* sigma_dev = nullptr, float sigma_scale = 1.0f) {
@@ -360,8 +381,22 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
             continue;
         }
         a.jobs = slot->d_jobs + c * cp.n_jobs;
+        hipEvent_t ev[3] = {};
+        rc = timing_events(p, ev);
+        if (rc) {
+            return rc;
+        }
+        if (ev[0]) {
+            HIP_TRY(hipEventRecord(ev[0], stream));
+        }
         HIP_TRY(launch_fused(a, key, static_cast<unsigned>(ch.items), ts.lds, stream));
+        if (ev[1]) {
+            HIP_TRY(hipEventRecord(ev[1], stream));
+        }
         HIP_TRY(launch_finalize(a, p->sample_code, ch.frames, ch.max_pixels, stream));
+        if (ev[2]) {
+            HIP_TRY(hipEventRecord(ev[2], stream));
+        }
     }
     return DFT2_OK;
 }
@@ -389,6 +424,10 @@ int make_job(const dft2_plan* p, const dft2_job& uj, const TileShape& ts, JobDev
     }
     if (uj.src_stride < static_cast<long long>(uj.width) * bps || uj.dst_stride < static_cast<long long>(uj.width) * bps) {
         return fail(DFT2_ERR_ARG, "row stride smaller than the row");
+    }
+    // The kernels address samples within one source frame with 32-bit byte offsets.
+    if (uj.src_stride * uj.height >= (1LL << 31)) {
+        return fail(DFT2_ERR_ARG, "a source frame must be smaller than 2 GiB");
     }
     std::memset(&j, 0, sizeof(j));
     j.src = uj.src;
@@ -552,6 +591,43 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
     return DFT2_OK;
 }
 
+int dft2_plan_set_timing(dft2_plan* p, int enable) {
+    clear_error();
+    if (!p) {
+        return fail(DFT2_ERR_ARG, "plan must not be NULL");
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (enable) {
+        // Events handed out earlier may still be pending; wait before they are reused.
+        for (size_t i = 0; i < p->timing_used; i++) {
+            HIP_TRY(hipEventSynchronize(p->timing_events[i]));
+        }
+        p->timing_used = 0;
+    }
+    p->timing = enable != 0;
+    return DFT2_OK;
+}
+
+int dft2_plan_get_timing(dft2_plan* p, dft2_kernel_timing* out) {
+    clear_error();
+    if (!p || !out) {
+        return fail(DFT2_ERR_ARG, "plan and out must not be NULL");
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    std::memset(out, 0, sizeof(*out));
+    for (size_t i = 0; i + 2 < p->timing_used; i += 3) {
+        HIP_TRY(hipEventSynchronize(p->timing_events[i + 2]));
+        float f = 0.0f, g = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&f, p->timing_events[i], p->timing_events[i + 1]));
+        HIP_TRY(hipEventElapsedTime(&g, p->timing_events[i + 1], p->timing_events[i + 2]));
+        out->fused_launches++;
+        out->fused_ms += f;
+        out->finalize_launches++;
+        out->finalize_ms += g;
+    }
+    return DFT2_OK;
+}
+
 int dft2_plan_destroy(dft2_plan* p) {
     if (!p) {
         return DFT2_OK;
@@ -586,6 +662,10 @@ int dft2_plan_destroy(dft2_plan* p) {
         if (s.stream) {
             (void)hipStreamDestroy(s.stream);
         }
+    }
+    for (hipEvent_t e : p->timing_events) {
+        (void)hipEventSynchronize(e);
+        (void)hipEventDestroy(e);
     }
     if (p->d_consts) {
         (void)hipFree(p->d_consts);

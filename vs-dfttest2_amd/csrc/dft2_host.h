@@ -96,6 +96,12 @@ struct dft2_plan {
     std::vector<dft2::host::Slot> slots;
     std::mutex mu;
     std::condition_variable cv;
+    // Kernel timing (dft2_plan_set_timing): events before the fused launch, between the
+    // fused and finalize launches and after the finalize launch, one triple per launch
+    // pair; guarded by mu.
+    bool timing = false;
+    std::vector<hipEvent_t> timing_events;
+    size_t timing_used = 0;
 };
 
 namespace dft2 {

@@ -27,6 +27,8 @@
 // (temporal) real matrix, and the complex 3-D coefficients are sums/differences of the
 // eight cos/sin products (see the middle stage below).
 
+#include <type_traits>
+
 #include "dft2_internal.h"
 
 namespace dft2 {
@@ -104,6 +106,62 @@ __device__ __forceinline__ int mirror(int v, int n) {
     return v >= n ? 2 * (n - 1) - v : v;
 }
 
+// Values that are the same in every lane of the wave, moved to scalar registers (the
+// compiler cannot prove that for values derived from threadIdx.x / 64 or read from the
+// job table, and would otherwise keep them, and everything computed from them, per lane).
+__device__ __forceinline__ int sgpr(int v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ long long sgpr64(long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<unsigned long long>(v) >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+template <typename P>
+__device__ __forceinline__ P* sgpr_ptr(P* p) {
+    return reinterpret_cast<P*>(sgpr64(reinterpret_cast<long long>(p)));
+}
+
+// Source samples are read with raw buffer loads: a descriptor per source frame (base,
+// size in bytes; loads past the size return 0 instead of faulting) and a 32-bit byte
+// offset per lane, plus an immediate offset per column.  The raw bits stay in a register
+// until the sample is used, so a prefetch keeps its loads in flight.
+constexpr int kBufferDword3 = 0x00020000; // gfx9-family descriptor word 3 (raw access)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), kBufferDword3);
+}
+
+// Raw sample as loaded: the integer sample itself, or the bits of a float sample.
+template <typename Tin>
+using raw_t = typename std::conditional<sizeof(Tin) == 4, unsigned, Tin>::type;
+
+template <typename Tin>
+__device__ __forceinline__ raw_t<Tin> load_raw(__amdgpu_buffer_rsrc_t r, unsigned off_bytes);
+template <>
+__device__ __forceinline__ uint8_t load_raw<uint8_t>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off_bytes, 0, 0);
+}
+template <>
+__device__ __forceinline__ uint16_t load_raw<uint16_t>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off_bytes, 0, 0);
+}
+template <>
+__device__ __forceinline__ unsigned load_raw<float>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0);
+}
+
+template <typename Tin>
+__device__ __forceinline__ float raw_value(raw_t<Tin> v) {
+    if constexpr (sizeof(Tin) == 4) {
+        return __builtin_bit_cast(float, v);
+    } else {
+        return static_cast<float>(v);
+    }
+}
+
 // Source planes live in device global memory; loading through an address-space-1
 // pointer gives global (not flat) loads.
 template <typename Tin>
@@ -119,7 +177,9 @@ __device__ __forceinline__ gptr_t<Tin> to_global(const Tin* p) {
 template <int FTM>
 __device__ __forceinline__ float gain_of(float psd, float s, const LaunchArgs& a) {
     if constexpr (FTM == 0) {
-        return fmaxf((psd - s) * __builtin_amdgcn_rcpf(psd + 1e-15f), 0.0f);
+        // s arrives as sigma + 1e-15 (gain table of a Wiener plan):
+        // (psd - sigma) / (psd + 1e-15) = 1 - (sigma + 1e-15) / (psd + 1e-15).
+        return fmaxf(fmaf(-s, __builtin_amdgcn_rcpf(psd + 1e-15f), 1.0f), 0.0f);
     } else if constexpr (FTM == 1) {
         return psd < s ? 0.0f : 1.0f;
     } else {
@@ -203,7 +263,9 @@ __device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, long lon
 }
 
 // Gain table into LDS: the plan's table, with sigma taken from a.sigma_dev when given.
-template <int R>
+// kBias: sigma + 1e-15 (the form gain_of<0> takes; the plan's table of a Wiener plan
+// holds it already) instead of sigma.
+template <int R, bool kBias>
 __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int tid, int n_threads) {
     constexpr int T = 2 * R + 1;
     const float4* gtab = reinterpret_cast<const float4*>(a.consts + kOffTAB);
@@ -214,6 +276,10 @@ __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int t
             const int3 pos = slot_position(g, slot, lane, T);
             e.x = a.sigma_dev[sigma_index(pos)] * a.sigma_scale;
             e.y = a.sigma_dev[sigma_index(partner_position(pos, T))] * a.sigma_scale;
+            if constexpr (kBias) {
+                e.x += 1e-15f;
+                e.y += 1e-15f;
+            }
         }
         tab[i] = e;
     }
@@ -227,7 +293,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = sgpr(tid >> 6);
     const int l16 = lane & 15;
     const int lg = lane >> 4;
 
@@ -251,7 +317,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     float* acc = smem;
     float4* tab = reinterpret_cast<float4*>(smem + a.region_h * rw);
     float* wbuf = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64) + wave * kWbuf;
-    fill_tab<R>(tab, a, tid, kThreads);
+    fill_tab<R, FTM == 0>(tab, a, tid, kThreads);
 
     // ---- per-lane constant operands ----
     float wx[4], wy[4], gx[4], gy[4], syn[4];
@@ -260,8 +326,12 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     for (int s = 0; s < 4; s++) {
         wx[s] = a.consts[kOffWX + (4 * lg + s) * 16 + l16];
         wy[s] = a.consts[kOffWY + l16 * 16 + 4 * lg + s];
-        gx[s] = a.consts[kOffGX + (4 * lg + s) * 16 + l16];
-        gy[s] = a.consts[kOffGY + l16 * 16 + 4 * lg + s];
+        // Inverse operands: the unweighted basis (WX, WY read transposed).  The middle
+        // stage leaves every coefficient of a non-degenerate pair at twice its value
+        // (the 1/2 of the complex <-> real basis maps is not applied), which is exactly
+        // the weight 2 of those basis functions in the inverse.
+        gx[s] = a.consts[kOffWX + l16 * 16 + 4 * lg + s];
+        gy[s] = a.consts[kOffWY + (4 * lg + s) * 16 + l16];
         syn[s] = a.consts[kOffSYN + (4 * lg + s) * 16 + l16];
 #pragma unroll
         for (int t = 0; t < T; t++) {
@@ -295,76 +365,117 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
 
     while (item < a.n_items) {
     // ---- work item -> (job, frame, tile) ----
-    const JobDev& job = a.jobs[find_job<false>(a.jobs, a.n_jobs, item)];
-    const int tiles = job.tiles_x * job.tiles_y;
-    const int local = item - static_cast<int>(job.item_begin);
+    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const int tiles_x = sgpr(job.tiles_x);
+    const int tiles = tiles_x * sgpr(job.tiles_y);
+    const int local = item - sgpr(static_cast<int>(job.item_begin));
     const int f_local = local / tiles;
     const int tile = local - f_local * tiles;
-    const int ty = tile / job.tiles_x;
-    const int tx = tile - ty * job.tiles_x;
-    const int n = job.f_begin + f_local;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int n = sgpr(job.f_begin) + f_local;
     const int by0 = ty * a.tile_bh;
     const int bx0 = tx * a.tile_bw;
-    const int nby = min(a.tile_bh, job.nby - by0);
-    const int nbx = min(a.tile_bw, job.nbx - bx0);
+    const int nby = min(a.tile_bh, sgpr(job.nby) - by0);
+    const int nbx = min(a.tile_bw, sgpr(job.nbx) - bx0);
     const int nblk = nby * nbx;
     const int rows_used = S * (nby - 1) + 16;
+    const int width = sgpr(job.width);
+    const int height = sgpr(job.height);
+    const int ox = sgpr(job.ox);
+    const int oy = sgpr(job.oy);
+    // Row stride in samples; a frame holds fewer than 2^31 samples (checked on the host),
+    // so sample offsets within a frame are 32-bit.
+    const int rs = sgpr(static_cast<int>(job.src_rs));
+    const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg);
 
     for (int i = tid; i < rows_used * rw; i += kThreads) {
         acc[i] = 0.0f;
     }
 
     // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
-    gptr_t<Tin> frame[T];
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-        int fi = min(max(n - R + t, 0), job.n_frames - 1);
-        frame[t] = to_global(job.use_fptr ? reinterpret_cast<const Tin*>(job.fptr[fi])
-                                          : reinterpret_cast<const Tin*>(job.src) + static_cast<long long>(fi) * job.src_fp);
-    }
-    // Samples of one block for this lane: row y = lane%16, columns 4*(lane/16)..+3 of
-    // every slice, with the mirror padding applied in the address.
-    auto fetch = [&](int blk, float (&dst)[T][4]) {
-        const int fy = blk / nbx;
-        const int fx = blk - fy * nbx;
-        const long long roff =
-            static_cast<long long>(mirror((by0 + fy) * S + l16 - job.oy, job.height)) * job.src_rs;
-        int cols[4];
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            cols[s] = mirror((bx0 + fx) * S + 4 * lg + s - job.ox, job.width);
-        }
+    __amdgpu_buffer_rsrc_t frame[T];
+    {
+        const int n_frames = sgpr(job.n_frames);
+        const bool use_fptr = sgpr(job.use_fptr) != 0;
+        const Tin* src = sgpr_ptr(reinterpret_cast<const Tin*>(job.src));
+        const long long fp = sgpr64(job.src_fp);
+        const unsigned frame_bytes = static_cast<unsigned>(rs * height) * static_cast<unsigned>(sizeof(Tin));
 #pragma unroll
         for (int t = 0; t < T; t++) {
+            const int fi = min(max(n - R + t, 0), n_frames - 1);
+            const Tin* base = use_fptr ? sgpr_ptr(reinterpret_cast<const Tin*>(job.fptr[fi]))
+                                       : src + static_cast<long long>(fi) * fp;
+            frame[t] = frame_rsrc(base, frame_bytes);
+        }
+    }
+    // Samples of block (fy, fx) of the tile for this lane: row y = lane%16, columns
+    // 4*(lane/16)..+3 of every slice, as raw bits.  Blocks that lie inside the plane
+    // (most of them) take one offset per lane; the others apply the mirror padding per
+    // sample.
+    constexpr unsigned kSz = sizeof(Tin);
+    auto fetch = [&](int fy, int fx, raw_t<Tin> (&dst)[T][4]) {
+        const int y0 = (by0 + fy) * S - oy;
+        const int x0 = (bx0 + fx) * S - ox;
+        if (y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width) {
+            const unsigned off = (static_cast<unsigned>(y0 * rs + x0) + lane_off) * kSz;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    dst[t][s] = load_raw<Tin>(frame[t], off + s * kSz);
+                }
+            }
+        } else {
+            const unsigned roff = static_cast<unsigned>(mirror(y0 + l16, height) * rs);
+            unsigned offs[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                dst[t][s] = static_cast<float>(frame[t][roff + cols[s]]);
+                offs[s] = (roff + static_cast<unsigned>(mirror(x0 + 4 * lg + s, width))) * kSz;
+            }
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    dst[t][s] = load_raw<Tin>(frame[t], offs[s]);
+                }
             }
         }
     };
 
     // The next block's samples are fetched one iteration ahead, so their load latency
     // overlaps the current block's transforms.
-    float nxt[T][4];
+    // Block b of the tile is (b / nbx, b % nbx); the wave's blocks advance by kWaves.
+    auto step_block = [&](int& by, int& bx) {
+        bx += kWaves;
+        while (bx >= nbx) {
+            bx -= nbx;
+            ++by;
+        }
+    };
+    int nby_ = wave / nbx;
+    int nbx_ = wave - nby_ * nbx;
+    raw_t<Tin> nxt[T][4];
     if (wave < nblk) {
-        fetch(wave, nxt);
+        fetch(nby_, nbx_, nxt);
     }
 
     __syncthreads();
 
     for (int b = wave; b < nblk; b += kWaves) {
-        const int bly = b / nbx;
-        const int blx = b - bly * nbx;
+        const int bly = nby_;
+        const int blx = nbx_;
         float cur[T][4];
 #pragma unroll
         for (int t = 0; t < T; t++) {
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                cur[t][s] = nxt[t][s];
+                cur[t][s] = raw_value<Tin>(nxt[t][s]);
             }
         }
+        step_block(nby_, nbx_);
         if (b + kWaves < nblk) {
-            fetch(b + kWaves, nxt);
+            fetch(nby_, nbx_, nxt);
         }
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
@@ -379,12 +490,35 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         }
 
         // ---- temporal real basis, staged as [j][p][q][b*2+c] ----
+        // cos(2 pi j t/T) is even and sin odd under t -> T - t, so the rows need only
+        // the sums and differences of the slice pairs (t, T - t).
+        floatx4 tsum[R + 1], tdif[R + 1]; // [t] for t = 1..R
+#pragma unroll
+        for (int t = 1; t <= R; t++) {
+            tsum[t] = d2[t] + d2[T - t];
+            tdif[t] = d2[t] - d2[T - t];
+        }
 #pragma unroll
         for (int j = 0; j < T; j++) {
-            floatx4 u = d2[0] * wt[j][0];
+            floatx4 u;
+            if (j == 0) {
+                u = d2[0];
 #pragma unroll
-            for (int t = 1; t < T; t++) {
-                u += d2[t] * wt[j][t];
+                for (int t = 1; t <= R; t++) {
+                    u += tsum[t];
+                }
+            } else if (j <= R) {
+                u = d2[0];
+#pragma unroll
+                for (int t = 1; t <= R; t++) {
+                    u += tsum[t] * wt[j][t];
+                }
+            } else if constexpr (R > 0) {
+                u = tdif[1] * wt[j][1];
+#pragma unroll
+                for (int t = 2; t <= R; t++) {
+                    u += tdif[t] * wt[j][t];
+                }
             }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -442,10 +576,11 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
                 float bi = ex ? -z0i - zxr : zxi;
                 filter_coef<FTM>(ar, ai, tab[(g * 4 + 2 * e) * 64 + lane], gf, a);
                 filter_coef<FTM>(br, bi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, a);
-                zr[e][0] = ex ? 0.5f * (ar + br) : ar;
-                zi[e][0] = ex ? 0.5f * (ai - bi) : ai;
-                zr[e][1] = ex ? -0.5f * (ai + bi) : br;
-                zi[e][1] = ex ? 0.5f * (ar - br) : bi;
+                // x slots back from (A, B), doubled for a proper pair (see gx above).
+                zr[e][0] = ex ? ar + br : ar;
+                zi[e][0] = ex ? ai - bi : ai;
+                zr[e][1] = ex ? -(ai + bi) : br;
+                zi[e][1] = ex ? ar - br : bi;
             }
 
             // Back to the real (t, y) basis, then the centre-slice inverse weights.
@@ -454,10 +589,11 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 const float pr = zr[0][c], pi = zi[0][c], qr = zr[1][c], qi = zi[1][c];
-                const float u_c0 = ey ? 0.5f * (pr + qr) : pr;   // cos t, y slot 0
-                const float u_s1 = ey ? 0.5f * (qr - pr) : -qi;  // sin t, y slot 1
-                const float u_s0 = ey ? -0.5f * (pi + qi) : -pi; // sin t, y slot 0
-                const float u_c1 = ey ? 0.5f * (qi - pi) : qr;   // cos t, y slot 1
+                // doubled for a proper y pair (see gy above)
+                const float u_c0 = ey ? pr + qr : pr;     // cos t, y slot 0
+                const float u_s1 = ey ? qr - pr : -qi;    // sin t, y slot 1
+                const float u_s0 = ey ? -(pi + qi) : -pi; // sin t, y slot 0
+                const float u_c1 = ey ? qi - pi : qr;     // cos t, y slot 1
                 const float y0 = wc_ * u_c0 + ws_ * u_s0;
                 const float y1 = wc_ * u_c1 + ws_ * u_s1;
                 if (c == 0) {
@@ -597,7 +733,7 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     float* gacc = dsig_lds + kSigPad;
     float* wbuf = gacc + T * region + wave * kWbuf;
     float* vbuf = wbuf + T * 256;
-    fill_tab<R>(tab, a, tid, kThreads);
+    fill_tab<R, false>(tab, a, tid, kThreads);
     for (int i = tid; i < T * 144; i += kThreads) {
         dsig_lds[i] = 0.0f;
     }

@@ -26,6 +26,8 @@ EXPORTED_SYMBOLS = (
     "dft2_process_batch",
     "dft2_process_batch_sigma",
     "dft2_backward_batch",
+    "dft2_plan_set_timing",
+    "dft2_plan_get_timing",
     "dft2_batch_create",
     "dft2_batch_launch",
     "dft2_batch_get_info",
@@ -90,6 +92,15 @@ class Dft2GradJob(ctypes.Structure):
         ("n_frames", ctypes.c_int),
         ("frame_begin", ctypes.c_int),
         ("frame_end", ctypes.c_int),
+    ]
+
+
+class Dft2KernelTiming(ctypes.Structure):
+    _fields_ = [
+        ("fused_launches", ctypes.c_longlong),
+        ("fused_ms", ctypes.c_double),
+        ("finalize_launches", ctypes.c_longlong),
+        ("finalize_ms", ctypes.c_double),
     ]
 
 
@@ -168,6 +179,10 @@ def load():
         ctypes.c_void_p, ctypes.c_void_p,
     ]
     lib.dft2_backward_batch.restype = ctypes.c_int
+    lib.dft2_plan_set_timing.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.dft2_plan_set_timing.restype = ctypes.c_int
+    lib.dft2_plan_get_timing.argtypes = [ctypes.c_void_p, ctypes.POINTER(Dft2KernelTiming)]
+    lib.dft2_plan_get_timing.restype = ctypes.c_int
     lib.dft2_batch_create.argtypes = [
         ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.POINTER(Dft2BatchOptions),
         ctypes.POINTER(ctypes.c_void_p),
@@ -257,6 +272,16 @@ class Plan:
             self.close()
         except Exception:
             pass
+
+    def set_timing(self, enable: bool):
+        """Records HIP events around every fused / finalize launch while enabled."""
+        _check(load().dft2_plan_set_timing(self._h, int(enable)))
+
+    def timing(self) -> dict:
+        """Summed fused / finalize kernel times (ms) and launch counts since set_timing(True)."""
+        out = Dft2KernelTiming()
+        _check(load().dft2_plan_get_timing(self._h, ctypes.byref(out)))
+        return {name: getattr(out, name) for name, _ in Dft2KernelTiming._fields_}
 
     def process_batch(self, jobs: List[Dft2Job], stream: Optional[int] = None):
         arr = (Dft2Job * len(jobs))(*jobs)
