@@ -374,7 +374,7 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         a.sigma_scale = sigma_scale;
         a.asym = 1;
     }
-    const FusedKey key = fused_key(p);
+    const FusedKey key = fused_key(p, 0, a.asym);
     for (size_t c = 0; c < cp.chunks.size(); c++) {
         const Chunk& ch = cp.chunks[c];
         if (ch.items == 0) {
@@ -765,7 +765,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
             return rc;
         }
         if (g.go_stride < static_cast<long long>(g.width) * 4 || g.go_stride % 4 || g.go_frame_pitch % 4 ||
-            g.gi_frame_pitch % 4) {
+            g.gi_frame_pitch % 4 || g.go_stride * g.height >= (1LL << 31)) {
             return fail(DFT2_ERR_ARG, "invalid gradient strides");
         }
         jd[i].grad_out = g.grad_out;

@@ -89,6 +89,7 @@ struct FusedKey {
     int precision;
     int filter_type;
     int persistent; // 1: persistent grid over the work queue (LaunchArgs::queue)
+    int asym;       // LaunchArgs::asym
 };
 
 // Sample type codes.

@@ -49,6 +49,22 @@ struct Opnd {
     bf16x4 hi, lo;
 };
 
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// BF16 hi part of (a, b) as one packed register, and the FP32 values of its halves (a
+// BF16 is the upper half of an FP32 with the same value).
+__device__ __forceinline__ unsigned pack_hi(float a, float b, float& ha, float& hb) {
+    const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector(floatx2{a, b}, bf16x2));
+    ha = __builtin_bit_cast(float, h << 16);
+    hb = __builtin_bit_cast(float, h & 0xffff0000u);
+    return h;
+}
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(floatx2{a, b}, bf16x2));
+}
+
 template <int PREC>
 __device__ __forceinline__ Opnd make_opnd(float a0, float a1, float a2, float a3) {
     Opnd o;
@@ -57,12 +73,14 @@ __device__ __forceinline__ Opnd make_opnd(float a0, float a1, float a2, float a3
     o.f[2] = a2;
     o.f[3] = a3;
     if constexpr (PREC == 1) {
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            const __bf16 h = static_cast<__bf16>(o.f[s]);
-            o.hi[s] = h;
-            o.lo[s] = static_cast<__bf16>(o.f[s] - static_cast<float>(h));
-        }
+        // Packed conversions: two values per instruction for hi and for lo.
+        float h0, h1, h2, h3;
+        const unsigned hi01 = pack_hi(a0, a1, h0, h1);
+        const unsigned hi23 = pack_hi(a2, a3, h2, h3);
+        const unsigned lo01 = pack_bf16(a0 - h0, a1 - h1);
+        const unsigned lo23 = pack_bf16(a2 - h2, a3 - h3);
+        o.hi = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(hi23) << 32) | hi01);
+        o.lo = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(lo23) << 32) | lo01);
     }
     return o;
 }
@@ -94,11 +112,13 @@ __device__ __forceinline__ floatx4 mm16(const Opnd& A, const Opnd& B, floatx4 ac
     return acc;
 }
 
-// Completes this wave's outstanding LDS operations and stops the compiler from moving
-// memory accesses across this point, so that LDS values written by one lane can be
-// read by another lane of the same wave afterwards.
+// Orders this wave's LDS accesses before and after this point, so that values one lane
+// writes can be read by another lane of the same wave afterwards.  The LDS executes a
+// wave's instructions in issue order, so only the compiler has to be kept from moving
+// accesses across the point; nothing waits here (the compiler still waits for a load's
+// data before its first use).
 __device__ __forceinline__ void wave_lds_sync() {
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __builtin_amdgcn_wave_barrier();
 }
 
 __device__ __forceinline__ int mirror(int v, int n) {
@@ -207,14 +227,19 @@ __device__ __forceinline__ float gain_of(float psd, float s, const LaunchArgs& a
 // the coefficients at kx = 0 and kx = 8 take the mean of their own gain and their
 // conjugate partner's gain, which is what the reference's C2R (it drops Im at those
 // columns) makes of two different gains.
-template <int FTM>
+// ASYM: 0 = symmetric sigma (no partner gains), 1 = asymmetric, 2 = decided at run time
+// from a.asym.  The forms are separate kernel instantiations because the compiler turns
+// a run-time test into selects and computes the partner gains for every plan.
+template <int FTM, int ASYM>
 __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, float gf, const LaunchArgs& a) {
     float xr = re - gf * e.z;
     float xi = im - gf * e.w;
     float psd = xr * xr + xi * xi;
     float g = gain_of<FTM>(psd, e.x, a);
-    if (a.asym && e.y != e.x) {  // only the kx = 0, 8 lanes of an asymmetric sigma array
-        g = 0.5f * (g + gain_of<FTM>(psd, e.y, a));
+    if constexpr (ASYM != 0) {
+        if ((ASYM == 1 || a.asym) && e.y != e.x) { // only the kx = 0, 8 lanes
+            g = 0.5f * (g + gain_of<FTM>(psd, e.y, a));
+        }
     }
     re = xr * g + gf * e.z;
     im = xi * g + gf * e.w;
@@ -285,7 +310,7 @@ __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int t
     }
 }
 
-template <int R, typename Tin, int FTM, int PREC, bool PERSIST>
+template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM>
 __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
     constexpr int kWbuf = (T * 256 > 320) ? T * 256 : 320;
@@ -574,8 +599,8 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
                 float ai = ex ? z0i - zxr : z0i;
                 float br = ex ? z0r - zxi : zxr;
                 float bi = ex ? -z0i - zxr : zxi;
-                filter_coef<FTM>(ar, ai, tab[(g * 4 + 2 * e) * 64 + lane], gf, a);
-                filter_coef<FTM>(br, bi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, a);
+                filter_coef<FTM, ASYM>(ar, ai, tab[(g * 4 + 2 * e) * 64 + lane], gf, a);
+                filter_coef<FTM, ASYM>(br, bi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, a);
                 // x slots back from (A, B), doubled for a proper pair (see gx above).
                 zr[e][0] = ex ? ar + br : ar;
                 zi[e][0] = ex ? ai - bi : ai;
@@ -703,25 +728,32 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
-    const int wave = tid >> 6;
+    const int wave = sgpr(tid >> 6);
     const int l16 = lane & 15;
     const int lg = lane >> 4;
 
-    const long long item = blockIdx.x;
-    const JobDev& job = a.jobs[find_job<false>(a.jobs, a.n_jobs, item)];
-    const int tiles = job.tiles_x * job.tiles_y;
-    const long long local = item - job.item_begin;
-    const int f_local = static_cast<int>(local / tiles);
-    const int tile = static_cast<int>(local - static_cast<long long>(f_local) * tiles);
-    const int ty = tile / job.tiles_x;
-    const int tx = tile - ty * job.tiles_x;
-    const int n = job.f_begin + f_local;
+    const int item = blockIdx.x;
+    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const int tiles_x = sgpr(job.tiles_x);
+    const int tiles = tiles_x * sgpr(job.tiles_y);
+    const int local = item - sgpr(static_cast<int>(job.item_begin));
+    const int f_local = local / tiles;
+    const int tile = local - f_local * tiles;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int n = sgpr(job.f_begin) + f_local;
     const int S = a.block_step;
     const int by0 = ty * a.tile_bh;
     const int bx0 = tx * a.tile_bw;
-    const int nby = min(a.tile_bh, job.nby - by0);
-    const int nbx = min(a.tile_bw, job.nbx - bx0);
+    const int nby = min(a.tile_bh, sgpr(job.nby) - by0);
+    const int nbx = min(a.tile_bw, sgpr(job.nbx) - bx0);
     const int nblk = nby * nbx;
+    const int width = sgpr(job.width);
+    const int height = sgpr(job.height);
+    const int ox = sgpr(job.ox);
+    const int oy = sgpr(job.oy);
+    const int rs = sgpr(static_cast<int>(job.src_rs));
+    const int go_rs = sgpr(static_cast<int>(job.go_rs));
 
     // LDS: gain table | sigma gradient | input-gradient regions [T][region_h][region_w] | per-wave staging
     const int rw = a.region_w;
@@ -741,15 +773,16 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         gacc[i] = 0.0f;
     }
 
-    float wx[4], wy[4], gxT[4], gyT[4], wyT[4], wxT[4], synA[4];
+    // The output-gradient chain uses the unweighted inverse basis (the transposes of
+    // which are WX and WY themselves), as the forward does: the adjoint middle stage then
+    // carries no 1/2 factors either.
+    float wx[4], wy[4], wyT[4], wxT[4], synA[4];
     float win[T][4];
     const float inv_scale = 1.0f / a.out_scale;
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         wx[s] = a.consts[kOffWX + (4 * lg + s) * 16 + l16];
         wy[s] = a.consts[kOffWY + l16 * 16 + 4 * lg + s];
-        gxT[s] = a.consts[kOffGX + l16 * 16 + 4 * lg + s];
-        gyT[s] = a.consts[kOffGY + (4 * lg + s) * 16 + l16];
         wyT[s] = a.consts[kOffWY + (4 * lg + s) * 16 + l16];
         wxT[s] = a.consts[kOffWX + l16 * 16 + 4 * lg + s];
         synA[s] = a.consts[kOffSYN + l16 * 16 + 4 * lg + s] * inv_scale;
@@ -760,8 +793,6 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     }
     const Opnd WXo = make_opnd<0>(wx[0], wx[1], wx[2], wx[3]);
     const Opnd WYo = make_opnd<0>(wy[0], wy[1], wy[2], wy[3]);
-    const Opnd GXTo = make_opnd<0>(gxT[0], gxT[1], gxT[2], gxT[3]);
-    const Opnd GYTo = make_opnd<0>(gyT[0], gyT[1], gyT[2], gyT[3]);
     const Opnd WYTo = make_opnd<0>(wyT[0], wyT[1], wyT[2], wyT[3]);
     const Opnd WXTo = make_opnd<0>(wxT[0], wxT[1], wxT[2], wxT[3]);
     float wt[T][T];
@@ -775,16 +806,27 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         }
     }
 
-    gptr_t<float> frame[T];
+    __amdgpu_buffer_rsrc_t frame[T];
     float* gin[T];
+    {
+        const int n_frames = sgpr(job.n_frames);
+        const float* src = sgpr_ptr(reinterpret_cast<const float*>(job.src));
+        float* gi = sgpr_ptr(reinterpret_cast<float*>(job.grad_in));
+        const long long fp = sgpr64(job.src_fp);
+        const long long gfp = sgpr64(job.gi_fp);
+        const unsigned frame_bytes = static_cast<unsigned>(rs * height) * 4u;
 #pragma unroll
-    for (int t = 0; t < T; t++) {
-        const int fi = min(max(n - R + t, 0), job.n_frames - 1);
-        frame[t] = to_global(reinterpret_cast<const float*>(job.src) + static_cast<long long>(fi) * job.src_fp);
-        gin[t] = reinterpret_cast<float*>(job.grad_in) + static_cast<long long>(fi) * job.gi_fp;
+        for (int t = 0; t < T; t++) {
+            const int fi = min(max(n - R + t, 0), n_frames - 1);
+            frame[t] = frame_rsrc(src + static_cast<long long>(fi) * fp, frame_bytes);
+            gin[t] = gi + static_cast<long long>(fi) * gfp;
+        }
     }
-    const gptr_t<float> gout = to_global(reinterpret_cast<const float*>(job.grad_out) +
-                                         static_cast<long long>(n) * job.go_fp);
+    const __amdgpu_buffer_rsrc_t gout =
+        frame_rsrc(sgpr_ptr(reinterpret_cast<const float*>(job.grad_out)) + static_cast<long long>(n) * sgpr64(job.go_fp),
+                   static_cast<unsigned>(go_rs * height) * 4u);
+    const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg) * 4u;
+    const unsigned lane_off_go = static_cast<unsigned>(l16 * go_rs + 4 * lg) * 4u;
 
     const int mp = lane >> 3;
     const int mq = lane & 7;
@@ -803,55 +845,111 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
 
     __syncthreads();
 
-    for (int b = wave; b < nblk; b += kWaves) {
-        const int bly = b / nbx;
-        const int blx = b - bly * nbx;
-        const int py = (by0 + bly) * S + l16;
-        const long long roff = static_cast<long long>(mirror(py - job.oy, job.height)) * job.src_rs;
-        int cols[4];
-#pragma unroll
-        for (int s = 0; s < 4; s++) {
-            cols[s] = mirror((bx0 + blx) * S + 4 * lg + s - job.ox, job.width);
+    int bly = wave / nbx;
+    int blx = wave - bly * nbx;
+    for (int b = wave; b < nblk; b += kWaves, blx += kWaves) {
+        while (blx >= nbx) {
+            blx -= nbx;
+            ++bly;
         }
+        const int y0 = (by0 + bly) * S - oy;
+        const int x0 = (bx0 + blx) * S - ox;
+        const bool inside = y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width;
         const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
 
         // ---- recompute the forward spectrum, staged as [j][p][q][b*2+c] ----
+        float smp[T][4];
+        if (inside) {
+            const unsigned off = static_cast<unsigned>(y0 * rs + x0) * 4u + lane_off;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    smp[t][s] = raw_value<float>(load_raw<float>(frame[t], off + 4u * s));
+                }
+            }
+        } else {
+            const unsigned roff = static_cast<unsigned>(mirror(y0 + l16, height) * rs);
+            unsigned offs[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                offs[s] = (roff + static_cast<unsigned>(mirror(x0 + 4 * lg + s, width))) * 4u;
+            }
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    smp[t][s] = raw_value<float>(load_raw<float>(frame[t], offs[s]));
+                }
+            }
+        }
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const Opnd in = make_opnd<0>(win[t][0] * frame[t][roff + cols[0]], win[t][1] * frame[t][roff + cols[1]],
-                                         win[t][2] * frame[t][roff + cols[2]], win[t][3] * frame[t][roff + cols[3]]);
+            const Opnd in = make_opnd<0>(win[t][0] * smp[t][0], win[t][1] * smp[t][1], win[t][2] * smp[t][2],
+                                         win[t][3] * smp[t][3]);
             const floatx4 d1 = mm16<0>(in, WXo, zero);
             d2[t] = mm16<0>(WYo, make_opnd<0>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
+        {
+            // temporal basis from the sums / differences of slices t and T - t
+            floatx4 tsum[R + 1], tdif[R + 1];
 #pragma unroll
-        for (int j = 0; j < T; j++) {
-            floatx4 u = d2[0] * wt[j][0];
-#pragma unroll
-            for (int t = 1; t < T; t++) {
-                u += d2[t] * wt[j][t];
+            for (int t = 1; t <= R; t++) {
+                tsum[t] = d2[t] + d2[T - t];
+                tdif[t] = d2[t] - d2[T - t];
             }
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int m = 4 * lg + i;
-                wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = u[i];
+            for (int j = 0; j < T; j++) {
+                floatx4 u = d2[0];
+                if (j == 0) {
+#pragma unroll
+                    for (int t = 1; t <= R; t++) {
+                        u += tsum[t];
+                    }
+                } else if (j <= R) {
+#pragma unroll
+                    for (int t = 1; t <= R; t++) {
+                        u += tsum[t] * wt[j][t];
+                    }
+                } else if constexpr (R > 0) {
+                    u = tdif[1] * wt[j][1];
+#pragma unroll
+                    for (int t = 2; t <= R; t++) {
+                        u += tdif[t] * wt[j][t];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int m = 4 * lg + i;
+                    wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = u[i];
+                }
             }
         }
 
-        // ---- output gradient -> dV = GY^T (dO GX^T), dO = dy / scale * synthesis window ----
+        // ---- output gradient -> dV = IY^T (dO IX^T), dO = dy / scale * synthesis window ----
         {
-            const int fy = py - job.oy;
-            const bool row_ok = fy >= 0 && fy < job.height;
             float dov[4];
+            if (inside) {
+                const unsigned off = static_cast<unsigned>(y0 * go_rs + x0) * 4u + lane_off_go;
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                const int fx = (bx0 + blx) * S + 4 * lg + s - job.ox;
-                dov[s] = (row_ok && fx >= 0 && fx < job.width)
-                             ? gout[static_cast<long long>(fy) * job.go_rs + fx] * synA[s]
-                             : 0.0f;
+                for (int s = 0; s < 4; s++) {
+                    dov[s] = raw_value<float>(load_raw<float>(gout, off + 4u * s)) * synA[s];
+                }
+            } else {
+                const int fy = y0 + l16;
+                const bool row_ok = fy >= 0 && fy < height;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int fx = x0 + 4 * lg + s;
+                    const bool ok = row_ok && fx >= 0 && fx < width;
+                    const float v = raw_value<float>(
+                        load_raw<float>(gout, ok ? static_cast<unsigned>(fy * go_rs + fx) * 4u : 0u));
+                    dov[s] = ok ? v * synA[s] : 0.0f;
+                }
             }
-            const floatx4 pm = mm16<0>(make_opnd<0>(dov[0], dov[1], dov[2], dov[3]), GXTo, zero);
-            const floatx4 dv = mm16<0>(GYTo, make_opnd<0>(pm[0], pm[1], pm[2], pm[3]), zero);
+            const floatx4 pm = mm16<0>(make_opnd<0>(dov[0], dov[1], dov[2], dov[3]), WXo, zero);
+            const floatx4 dv = mm16<0>(WYo, make_opnd<0>(pm[0], pm[1], pm[2], pm[3]), zero);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int m = 4 * lg + i;
@@ -906,19 +1004,19 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
             for (int c = 0; c < 2; c++) {
                 const float duc0 = wcg * dv0[c], dus0 = wsg * dv0[c];
                 const float duc1 = wcg * dv1[c], dus1 = wsg * dv1[c];
-                dzr[0][c] = ey ? 0.5f * (duc0 - dus1) : duc0;
-                dzr[1][c] = ey ? 0.5f * (duc0 + dus1) : duc1;
-                dzi[0][c] = ey ? -0.5f * (dus0 + duc1) : -dus0;
-                dzi[1][c] = ey ? 0.5f * (duc1 - dus0) : -dus1;
+                dzr[0][c] = ey ? duc0 - dus1 : duc0;
+                dzr[1][c] = ey ? duc0 + dus1 : duc1;
+                dzi[0][c] = ey ? -(dus0 + duc1) : -dus0;
+                dzi[1][c] = ey ? duc1 - dus0 : -dus1;
             }
             float gz1r[2], gz1i[2], gz2r[2], gz2i[2]; // gradients of Z (pre gain), [c]
 #pragma unroll
             for (int e = 0; e < 2; e++) {
                 const float d0r = dzr[e][0], d0i = dzi[e][0], d1r = dzr[e][1], d1i = dzi[e][1];
-                float dar = ex ? 0.5f * (d0r + d1i) : d0r;
-                float dai = ex ? 0.5f * (d0i - d1r) : d0i;
-                float dbr = ex ? 0.5f * (d0r - d1i) : d1r;
-                float dbi = ex ? 0.5f * (-d0i - d1r) : d1i;
+                float dar = ex ? d0r + d1i : d0r;
+                float dai = ex ? d0i - d1r : d0i;
+                float dbr = ex ? d0r - d1i : d1r;
+                float dbi = ex ? -d0i - d1r : d1i;
                 gain_adjoint(ar[e], ai[e], dar, dai, tab[(g * 4 + 2 * e) * 64 + lane], gf, dsg[g][2 * e], dgf);
                 gain_adjoint(br[e], bi[e], dbr, dbi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, dsg[g][2 * e + 1],
                              dgf);
@@ -966,13 +1064,34 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
                 du[j][i] = wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc];
             }
         }
+        // adjoint temporal basis: slice t gets C_t + S_t and slice T - t gets C_t - S_t
+        // (C: cos rows, even in t; S: sin rows, odd in t)
+        floatx4 ddt[T];
+        ddt[0] = du[0];
+#pragma unroll
+        for (int j = 1; j <= R; j++) {
+            ddt[0] += du[j];
+        }
+        if constexpr (R > 0) {
+#pragma unroll
+            for (int t = 1; t <= R; t++) {
+                floatx4 c = du[0];
+#pragma unroll
+                for (int j = 1; j <= R; j++) {
+                    c += du[j] * wt[j][t];
+                }
+                floatx4 sn = du[R + 1] * wt[R + 1][t];
+#pragma unroll
+                for (int k = 2; k <= R; k++) {
+                    sn += du[R + k] * wt[R + k][t];
+                }
+                ddt[t] = c + sn;
+                ddt[T - t] = c - sn;
+            }
+        }
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            floatx4 dd = du[0] * wt[0][t];
-#pragma unroll
-            for (int j = 1; j < T; j++) {
-                dd += du[j] * wt[j][t];
-            }
+            const floatx4 dd = ddt[t];
             // dd (m in registers) used as A operand is dD2^T; R1 = dD2^T WY, db^T = WX R1.
             const floatx4 r1 = mm16<0>(make_opnd<0>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
             const floatx4 dbt = mm16<0>(WXTo, make_opnd<0>(r1[0], r1[1], r1[2], r1[3]), zero);
@@ -1081,32 +1200,35 @@ __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
 }
 
 template <int R, typename Tin, int PREC, bool PERSIST>
-const void* fused_ptr_ft(int filter_type) {
-    // Gains: 0 Wiener, 1 hard threshold, 2 the generic form for types 2..6.
+const void* fused_ptr_ft(int filter_type, bool asym) {
+    // Gains: 0 Wiener, 1 hard threshold (each with a symmetric and an asymmetric form),
+    // 2 the generic form for types 2..6.
     if (filter_type == 0) {
-        return reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST>);
+        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 1>)
+                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 0>);
     }
     if (filter_type == 1) {
-        return reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST>);
+        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 1>)
+                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 0>);
     }
-    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, PERSIST>);
+    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, PERSIST, 2>);
 }
 
 template <int R, typename Tin, bool PERSIST>
-const void* fused_ptr_prec(int precision, int filter_type) {
-    return precision == 1 ? fused_ptr_ft<R, Tin, 1, PERSIST>(filter_type)
-                          : fused_ptr_ft<R, Tin, 0, PERSIST>(filter_type);
+const void* fused_ptr_prec(const FusedKey& k) {
+    return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, PERSIST>(k.filter_type, k.asym != 0)
+                            : fused_ptr_ft<R, Tin, 0, PERSIST>(k.filter_type, k.asym != 0);
 }
 
 template <int R, bool PERSIST>
-const void* fused_ptr_type(int sample_code, int precision, int filter_type) {
-    switch (sample_code) {
+const void* fused_ptr_type(const FusedKey& k) {
+    switch (k.sample_code) {
     case kU8:
-        return fused_ptr_prec<R, uint8_t, PERSIST>(precision, filter_type);
+        return fused_ptr_prec<R, uint8_t, PERSIST>(k);
     case kU16:
-        return fused_ptr_prec<R, uint16_t, PERSIST>(precision, filter_type);
+        return fused_ptr_prec<R, uint16_t, PERSIST>(k);
     default:
-        return fused_ptr_prec<R, float, PERSIST>(precision, filter_type);
+        return fused_ptr_prec<R, float, PERSIST>(k);
     }
 }
 
@@ -1114,13 +1236,13 @@ template <bool PERSIST>
 const void* fused_ptr_radius(const FusedKey& k) {
     switch (k.radius) {
     case 0:
-        return fused_ptr_type<0, PERSIST>(k.sample_code, k.precision, k.filter_type);
+        return fused_ptr_type<0, PERSIST>(k);
     case 1:
-        return fused_ptr_type<1, PERSIST>(k.sample_code, k.precision, k.filter_type);
+        return fused_ptr_type<1, PERSIST>(k);
     case 2:
-        return fused_ptr_type<2, PERSIST>(k.sample_code, k.precision, k.filter_type);
+        return fused_ptr_type<2, PERSIST>(k);
     default:
-        return fused_ptr_type<3, PERSIST>(k.sample_code, k.precision, k.filter_type);
+        return fused_ptr_type<3, PERSIST>(k);
     }
 }
 
@@ -1176,7 +1298,7 @@ size_t fused_lds_bytes(int radius, int region_w, int region_h) {
 
 hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, size_t lds_bytes,
                         hipStream_t stream) {
-    if (k.persistent != (a.queue != nullptr)) {
+    if (k.persistent != (a.queue != nullptr) || k.asym != a.asym) {
         return hipErrorInvalidValue;
     }
     const void* kern = fused_ptr(k);
