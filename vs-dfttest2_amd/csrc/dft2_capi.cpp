@@ -255,15 +255,33 @@ LaunchArgs make_args(const dft2_plan* p, const TileShape& ts) {
     return a;
 }
 
-// Stages `table_bytes` of job descriptors through the slot's pinned ring into d_jobs.
-int upload_jobs(Slot* slot, const JobDev* src_table, size_t table_bytes, hipStream_t stream) {
-    if (slot->d_jobs_cap < table_bytes) {
+// Makes `table_bytes` of job descriptors resident in device memory: a cached table with
+// the same bytes is reused; otherwise the least recently used cache entry is refilled
+// through the slot's pinned ring.  *out_dev receives the device table.
+int upload_jobs(Slot* slot, const JobDev* src_table, size_t table_bytes, hipStream_t stream, JobDev** out_dev) {
+    const unsigned char* src_bytes = reinterpret_cast<const unsigned char*>(src_table);
+    DeviceTable* victim = &slot->tables[0];
+    for (auto& t : slot->tables) {
+        if (t.dev && t.bytes.size() == table_bytes && std::memcmp(t.bytes.data(), src_bytes, table_bytes) == 0) {
+            t.last_use = ++slot->table_clock;
+            *out_dev = t.dev;
+            return DFT2_OK;
+        }
+        if (t.last_use < victim->last_use) {
+            victim = &t;
+        }
+    }
+    if (victim->cap < table_bytes) {
+        // Kernels queued earlier may still read the old buffer.
         HIP_TRY(hipStreamSynchronize(stream));
-        int rc = ensure_device(reinterpret_cast<void**>(&slot->d_jobs), &slot->d_jobs_cap, table_bytes);
+        int rc = ensure_device(reinterpret_cast<void**>(&victim->dev), &victim->cap, table_bytes);
         if (rc) {
             return rc;
         }
     }
+    victim->bytes.assign(src_bytes, src_bytes + table_bytes);
+    victim->last_use = ++slot->table_clock;
+    *out_dev = victim->dev;
     const int ring = slot->ring_pos;
     slot->ring_pos = (slot->ring_pos + 1) % kRing;
     if (slot->h_jobs_done[ring]) {
@@ -276,7 +294,7 @@ int upload_jobs(Slot* slot, const JobDev* src_table, size_t table_bytes, hipStre
         return rc;
     }
     std::memcpy(slot->h_jobs[ring], src_table, table_bytes);
-    HIP_TRY(hipMemcpyAsync(slot->d_jobs, slot->h_jobs[ring], table_bytes, hipMemcpyHostToDevice, stream));
+    HIP_TRY(hipMemcpyAsync(victim->dev, slot->h_jobs[ring], table_bytes, hipMemcpyHostToDevice, stream));
     HIP_TRY(hipEventRecord(slot->h_jobs_done[ring], stream));
     return DFT2_OK;
 }
@@ -362,7 +380,8 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
     }
     // Device job table: reused across calls on this slot; stream order (or the wait on
     // the slot's previous completion) keeps earlier kernels from seeing new data.
-    rc = upload_jobs(slot, cp.table.data(), sizeof(JobDev) * cp.table.size(), stream);
+    JobDev* d_table = nullptr;
+    rc = upload_jobs(slot, cp.table.data(), sizeof(JobDev) * cp.table.size(), stream, &d_table);
     if (rc) {
         return rc;
     }
@@ -380,7 +399,7 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         if (ch.items == 0) {
             continue;
         }
-        a.jobs = slot->d_jobs + c * cp.n_jobs;
+        a.jobs = d_table + c * cp.n_jobs;
         hipEvent_t ev[3] = {};
         rc = timing_events(p, ev);
         if (rc) {
@@ -653,8 +672,10 @@ int dft2_plan_destroy(dft2_plan* p) {
         if (s.scratch) {
             (void)hipFree(s.scratch);
         }
-        if (s.d_jobs) {
-            (void)hipFree(s.d_jobs);
+        for (auto& t : s.tables) {
+            if (t.dev) {
+                (void)hipFree(t.dev);
+            }
         }
         if (s.stage) {
             (void)hipFree(s.stage);
@@ -792,12 +813,13 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     if (stream) {
         HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     }
-    rc = upload_jobs(slot, jd.data(), sizeof(JobDev) * jd.size(), st);
+    JobDev* d_table = nullptr;
+    rc = upload_jobs(slot, jd.data(), sizeof(JobDev) * jd.size(), st, &d_table);
     if (rc) {
         return rc;
     }
     LaunchArgs a = make_args(p, p->tile);
-    a.jobs = slot->d_jobs;
+    a.jobs = d_table;
     a.n_jobs = n_jobs;
     a.sigma_dev = sigma_dev;
     a.sigma_scale = sigma_scale;

@@ -32,6 +32,15 @@ int pad_size(int size, int bs, int step);
 int pad_num(int size, int bs, int step);
 
 constexpr int kRing = 8;
+constexpr int kTableCache = 4;
+
+// A job table resident in device memory, with the host bytes it was made from.
+struct DeviceTable {
+    std::vector<unsigned char> bytes;
+    JobDev* dev = nullptr;
+    size_t cap = 0;
+    unsigned long long last_use = 0;
+};
 
 // One concurrent call's stream and buffers (num_streams of them per plan).
 struct Slot {
@@ -39,8 +48,10 @@ struct Slot {
     hipEvent_t done = nullptr;
     float* scratch = nullptr;
     size_t scratch_bytes = 0;
-    JobDev* d_jobs = nullptr;
-    size_t d_jobs_cap = 0;
+    // Recently uploaded job tables: a call whose table matches one of them byte for
+    // byte (the same buffers and geometry as an earlier call) reuses it without a copy.
+    DeviceTable tables[kTableCache];
+    unsigned long long table_clock = 0;
     JobDev* h_jobs[kRing] = {};
     size_t h_jobs_cap[kRing] = {};
     hipEvent_t h_jobs_done[kRing] = {};

@@ -722,6 +722,10 @@ __device__ __forceinline__ void gain_adjoint(float ar, float ai, float& dyr, flo
 template <int R>
 __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
+    // Contractions in split BF16 (hi + lo operands, three products, FP32 accumulation):
+    // about 16 significant bits per operand, well inside the gradients' 1e-3 bound, at a
+    // fraction of the FP32 MFMA cost.
+    constexpr int kBP = 1;
     constexpr int kWbuf = T * 256 + 256;
     constexpr int kSigPad = (T * 144 + 3) / 4 * 4;
     extern __shared__ float smem[];
@@ -791,10 +795,10 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
             win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
         }
     }
-    const Opnd WXo = make_opnd<0>(wx[0], wx[1], wx[2], wx[3]);
-    const Opnd WYo = make_opnd<0>(wy[0], wy[1], wy[2], wy[3]);
-    const Opnd WYTo = make_opnd<0>(wyT[0], wyT[1], wyT[2], wyT[3]);
-    const Opnd WXTo = make_opnd<0>(wxT[0], wxT[1], wxT[2], wxT[3]);
+    const Opnd WXo = make_opnd<kBP>(wx[0], wx[1], wx[2], wx[3]);
+    const Opnd WYo = make_opnd<kBP>(wy[0], wy[1], wy[2], wy[3]);
+    const Opnd WYTo = make_opnd<kBP>(wyT[0], wyT[1], wyT[2], wyT[3]);
+    const Opnd WXTo = make_opnd<kBP>(wxT[0], wxT[1], wxT[2], wxT[3]);
     float wt[T][T];
     float ct[T];
 #pragma unroll
@@ -886,10 +890,10 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const Opnd in = make_opnd<0>(win[t][0] * smp[t][0], win[t][1] * smp[t][1], win[t][2] * smp[t][2],
+            const Opnd in = make_opnd<kBP>(win[t][0] * smp[t][0], win[t][1] * smp[t][1], win[t][2] * smp[t][2],
                                          win[t][3] * smp[t][3]);
-            const floatx4 d1 = mm16<0>(in, WXo, zero);
-            d2[t] = mm16<0>(WYo, make_opnd<0>(d1[0], d1[1], d1[2], d1[3]), zero);
+            const floatx4 d1 = mm16<kBP>(in, WXo, zero);
+            d2[t] = mm16<kBP>(WYo, make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
         {
             // temporal basis from the sums / differences of slices t and T - t
@@ -948,8 +952,8 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
                     dov[s] = ok ? v * synA[s] : 0.0f;
                 }
             }
-            const floatx4 pm = mm16<0>(make_opnd<0>(dov[0], dov[1], dov[2], dov[3]), WXo, zero);
-            const floatx4 dv = mm16<0>(WYo, make_opnd<0>(pm[0], pm[1], pm[2], pm[3]), zero);
+            const floatx4 pm = mm16<kBP>(make_opnd<kBP>(dov[0], dov[1], dov[2], dov[3]), WXo, zero);
+            const floatx4 dv = mm16<kBP>(WYo, make_opnd<kBP>(pm[0], pm[1], pm[2], pm[3]), zero);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int m = 4 * lg + i;
@@ -1093,8 +1097,8 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         for (int t = 0; t < T; t++) {
             const floatx4 dd = ddt[t];
             // dd (m in registers) used as A operand is dD2^T; R1 = dD2^T WY, db^T = WX R1.
-            const floatx4 r1 = mm16<0>(make_opnd<0>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
-            const floatx4 dbt = mm16<0>(WXTo, make_opnd<0>(r1[0], r1[1], r1[2], r1[3]), zero);
+            const floatx4 r1 = mm16<kBP>(make_opnd<kBP>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
+            const floatx4 dbt = mm16<kBP>(WXTo, make_opnd<kBP>(r1[0], r1[1], r1[2], r1[3]), zero);
             float* arow = gacc + t * region + (bly * S + l16) * rw + blx * S + 4 * lg;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
