@@ -227,6 +227,18 @@ __device__ __forceinline__ float gain_of(float psd, float s, const LaunchArgs& a
 // the coefficients at kx = 0 and kx = 8 take the mean of their own gain and their
 // conjugate partner's gain, which is what the reference's C2R (it drops Im at those
 // columns) makes of two different gains.
+// One gain-table entry from LDS as a single 128-bit read.  Without the empty asm the
+// symmetric form, which does not use the partner sigma, gets split 32/64-bit reads at a
+// 16-byte lane stride: twice the LDS instructions, with bank conflicts.
+template <int ASYM>
+__device__ __forceinline__ float4 tab_entry(const float4* p) {
+    float4 e = *p;
+    if constexpr (ASYM == 0) {
+        asm volatile("" : "+v"(e.y));
+    }
+    return e;
+}
+
 // ASYM: 0 = symmetric sigma (no partner gains), 1 = asymmetric, 2 = decided at run time
 // from a.asym.  The forms are separate kernel instantiations because the compiler turns
 // a run-time test into selects and computes the partner gains for every plan.
@@ -599,8 +611,8 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
                 float ai = ex ? z0i - zxr : z0i;
                 float br = ex ? z0r - zxi : zxr;
                 float bi = ex ? -z0i - zxr : zxi;
-                filter_coef<FTM, ASYM>(ar, ai, tab[(g * 4 + 2 * e) * 64 + lane], gf, a);
-                filter_coef<FTM, ASYM>(br, bi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, a);
+                filter_coef<FTM, ASYM>(ar, ai, tab_entry<ASYM>(&tab[(g * 4 + 2 * e) * 64 + lane]), gf, a);
+                filter_coef<FTM, ASYM>(br, bi, tab_entry<ASYM>(&tab[(g * 4 + 2 * e + 1) * 64 + lane]), gf, a);
                 // x slots back from (A, B), doubled for a proper pair (see gx above).
                 zr[e][0] = ex ? ar + br : ar;
                 zi[e][0] = ex ? ai - bi : ai;
