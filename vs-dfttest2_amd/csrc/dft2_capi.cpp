@@ -250,6 +250,7 @@ LaunchArgs make_args(const dft2_plan* p, const TileShape& ts) {
     a.sigma_dev = nullptr;
     a.sigma_scale = 1.0f;
     a.sigma_grad = nullptr;
+    a.tab_dev = nullptr;
     a.queue = nullptr;
     a.n_items = 0;
     return a;
@@ -365,6 +366,22 @@ int timing_events(dft2_plan* p, hipEvent_t ev[3]) {
     return DFT2_OK;
 }
 
+// Builds the launch's gain table from a.sigma_dev into the slot's buffer and points
+// a.tab_dev at it.
+int build_gain_table(const dft2_plan* p, Slot* slot, LaunchArgs& a, bool bias, hipStream_t stream) {
+    const size_t bytes = sizeof(float) * kTabFloats(p->radius);
+    if (slot->tab_dev_cap < bytes) {
+        int rc = ensure_device(reinterpret_cast<void**>(&slot->tab_dev), &slot->tab_dev_cap, bytes);
+        if (rc) {
+            return rc;
+        }
+    }
+    a.tab_dev = nullptr;
+    HIP_TRY(launch_gain_table(a, p->radius, bias, slot->tab_dev, stream));
+    a.tab_dev = slot->tab_dev;
+    return DFT2_OK;
+}
+
 // Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
 int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
              const float* sigma_dev = nullptr, float sigma_scale = 1.0f) {
@@ -392,6 +409,10 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         a.sigma_dev = sigma_dev;
         a.sigma_scale = sigma_scale;
         a.asym = 1;
+        rc = build_gain_table(p, slot, a, p->filter_type == 0, stream);
+        if (rc) {
+            return rc;
+        }
     }
     const FusedKey key = fused_key(p, 0, a.asym);
     for (size_t c = 0; c < cp.chunks.size(); c++) {
@@ -677,6 +698,9 @@ int dft2_plan_destroy(dft2_plan* p) {
                 (void)hipFree(t.dev);
             }
         }
+        if (s.tab_dev) {
+            (void)hipFree(s.tab_dev);
+        }
         if (s.stage) {
             (void)hipFree(s.stage);
         }
@@ -829,6 +853,10 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     a.tile_bh = tbh;
     a.region_w = region_w;
     a.region_h = region_h;
+    rc = build_gain_table(p, slot, a, false, st);
+    if (rc) {
+        return rc;
+    }
     HIP_TRY(launch_backward(a, p->radius, items, backward_lds_bytes(p->radius, region_w, region_h), st));
     HIP_TRY(hipEventRecord(slot->done, st));
     if (!stream) {

@@ -56,6 +56,8 @@ struct Slot {
     size_t h_jobs_cap[kRing] = {};
     hipEvent_t h_jobs_done[kRing] = {};
     int ring_pos = 0;
+    float4* tab_dev = nullptr; // gain table built per launch when sigma comes from a device array
+    size_t tab_dev_cap = 0;
     void* stage = nullptr; // host-memory path: T input planes + output plane
     size_t stage_bytes = 0;
     bool busy = false;

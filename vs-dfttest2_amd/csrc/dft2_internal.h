@@ -75,6 +75,9 @@ struct LaunchArgs {
     const float* sigma_dev;
     float sigma_scale;
     float* sigma_grad;          // backward: d loss / d sigma_dev, accumulated with +=
+    // Gain table of this launch in global memory ((radius+1)*4*64 entries, see kOffTAB),
+    // built by launch_gain_table from sigma_dev; nullptr = build per workgroup.
+    const float4* tab_dev;
     // Persistent launch: work items [0, n_items) are taken from the counter *queue
     // (zeroed before the launch); nullptr = one work item per workgroup.  A launch has
     // fewer than 2^31 work items.
@@ -101,6 +104,9 @@ hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, s
                         hipStream_t stream);
 // Workgroups of the fused kernel resident per CU with `lds_bytes` of LDS (0 on error).
 int fused_workgroups_per_cu(const FusedKey& k, size_t lds_bytes);
+// Builds the gain table of a launch with sigma from a.sigma_dev into `out` (device,
+// kTabFloats(radius) floats); bias: sigma + 1e-15 as the Wiener forward gain takes it.
+hipError_t launch_gain_table(const LaunchArgs& a, int radius, bool bias, float4* out, hipStream_t stream);
 // Zeroes n work-queue counters with a kernel (safe inside graph capture).
 hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream);
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,

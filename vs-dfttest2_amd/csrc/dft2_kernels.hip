@@ -305,6 +305,12 @@ __device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, long lon
 template <int R, bool kBias>
 __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int tid, int n_threads) {
     constexpr int T = 2 * R + 1;
+    if (a.tab_dev) { // built for this launch by gain_table_kernel
+        for (int i = tid; i < (R + 1) * 4 * 64; i += n_threads) {
+            tab[i] = a.tab_dev[i];
+        }
+        return;
+    }
     const float4* gtab = reinterpret_cast<const float4*>(a.consts + kOffTAB);
     for (int i = tid; i < (R + 1) * 4 * 64; i += n_threads) {
         float4 e = gtab[i];
@@ -320,6 +326,14 @@ __device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int t
         }
         tab[i] = e;
     }
+}
+
+// The gain table of one launch with sigma taken from a.sigma_dev, built once into global
+// memory (every workgroup then copies it instead of gathering sigma itself).
+template <int R, bool kBias>
+__global__ __launch_bounds__(256) void gain_table_kernel(LaunchArgs a, float4* out) {
+    a.tab_dev = nullptr;
+    fill_tab<R, kBias>(out, a, blockIdx.x * 256 + threadIdx.x, gridDim.x * 256);
 }
 
 template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM>
@@ -1121,16 +1135,40 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     }
 
     __syncthreads();
-    // input gradient: padded tile regions -> source samples (mirror, temporal clamp)
-    for (int t = 0; t < T; t++) {
+    // input gradient: padded tile regions -> source samples (mirror, temporal clamp).
+    // Region element i = (ry, rx) is stepped without a division; tiles whose region lies
+    // inside the plane skip the mirror.
+    {
+        const int gy0 = by0 * S - oy;
+        const int gx0 = bx0 * S - ox;
+        const bool inside = gy0 >= 0 && gy0 + rows_used <= height && gx0 >= 0 && gx0 + cols_used <= width;
+        const long long gi_rs = sgpr64(job.gi_rs);
+        const int dy = kThreads / rw;
+        const int dx = kThreads - dy * rw;
+        int ry = tid / rw;
+        int rx = tid - ry * rw;
         for (int i = tid; i < rows_used * rw; i += kThreads) {
-            const int ry = i / rw;
-            const int rx = i - ry * rw;
-            const float v = gacc[t * region + i];
-            if (rx < cols_used && v != 0.0f) {
-                const int fy = mirror(by0 * S + ry - job.oy, job.height);
-                const int fx = mirror(bx0 * S + rx - job.ox, job.width);
-                atomicAdd(gin[t] + static_cast<long long>(fy) * job.gi_rs + fx, v);
+            if (rx < cols_used) {
+                int fy = gy0 + ry;
+                int fx = gx0 + rx;
+                if (!inside) {
+                    fy = mirror(fy, height);
+                    fx = mirror(fx, width);
+                }
+                const long long off = static_cast<long long>(fy) * gi_rs + fx;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const float v = gacc[t * region + i];
+                    if (v != 0.0f) {
+                        atomicAdd(gin[t] + off, v);
+                    }
+                }
+            }
+            rx += dx;
+            ry += dy;
+            if (rx >= rw) {
+                rx -= rw;
+                ++ry;
             }
         }
     }
@@ -1346,6 +1384,23 @@ __global__ void queue_reset_kernel(unsigned* q, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         q[i] = 0;
+    }
+}
+
+hipError_t launch_gain_table(const LaunchArgs& a, int radius, bool bias, float4* out, hipStream_t stream) {
+    auto run = [&](auto kern) -> hipError_t {
+        hipLaunchKernelGGL(kern, dim3(4), dim3(256), 0, stream, a, out);
+        return hipGetLastError();
+    };
+    switch (radius) {
+    case 0:
+        return bias ? run(gain_table_kernel<0, true>) : run(gain_table_kernel<0, false>);
+    case 1:
+        return bias ? run(gain_table_kernel<1, true>) : run(gain_table_kernel<1, false>);
+    case 2:
+        return bias ? run(gain_table_kernel<2, true>) : run(gain_table_kernel<2, false>);
+    default:
+        return bias ? run(gain_table_kernel<3, true>) : run(gain_table_kernel<3, false>);
     }
 }
 
