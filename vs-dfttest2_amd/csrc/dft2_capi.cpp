@@ -61,6 +61,27 @@ bool tile_valid(const dft2_plan* p, int bw, int bh) {
     return bw >= need && bh >= need && bw <= 64 && bh <= 64;
 }
 
+TileShape choose_tile(const dft2_plan* p) {
+    static const int kCand[][2] = {{16, 16}, {12, 16}, {16, 12}, {12, 12}, {8, 16}, {16, 8}, {8, 12}, {8, 8}};
+    const int tb_max = 84 / p->block_step + 1; // accumulator region at most ~100 samples per side
+    TileShape best = make_tile(p, std::min(16, tb_max), std::min(16, tb_max));
+    long long best_score = -1;
+    for (const auto& c : kCand) {
+        const int bw = std::min(c[0], tb_max), bh = std::min(c[1], tb_max);
+        if (!tile_valid(p, bw, bh)) {
+            continue;
+        }
+        const TileShape ts = make_tile(p, bw, bh);
+        const int wg = fused_workgroups_per_cu(fused_key(p), ts.lds);
+        const long long score = static_cast<long long>(wg) * bw * bh;
+        if (wg > 0 && score > best_score) {
+            best_score = score;
+            best = ts;
+        }
+    }
+    return best;
+}
+
 TileShape make_tile(const dft2_plan* p, int bw, int bh) {
     TileShape ts;
     ts.bw = bw;
@@ -145,6 +166,58 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
                     e4[1] = (edge ? sigma_at((T - kt) % T, (16 - ky2) % 16, kx) : sigma_at(kt, ky2, kx)) + bias;
                     e4[2] = wf_at(kt, ky2, kx, 0);
                     e4[3] = wf_at(kt, ky2, kx, 1);
+                }
+            }
+        }
+    }
+    // Separable window: w[t][y][x] = a[t] b[y] c[x] to within 1e-9 of its peak.
+    {
+        const double* w = prm->window;
+        int t0 = 0, y0 = 0, x0 = 0;
+        double peak = 0.0;
+        for (int i = 0; i < T * 256; i++) {
+            if (std::fabs(w[i]) > peak) {
+                peak = std::fabs(w[i]);
+                t0 = i / 256, y0 = (i / 16) % 16, x0 = i % 16;
+            }
+        }
+        p->separable = 0;
+        if (peak > 0.0) {
+            const double w0 = w[(t0 * 16 + y0) * 16 + x0];
+            double at[7], by[16], cx[16];
+            for (int t = 0; t < T; t++) {
+                at[t] = w[(t * 16 + y0) * 16 + x0];
+            }
+            for (int i = 0; i < 16; i++) {
+                by[i] = w[(t0 * 16 + i) * 16 + x0] / w0;
+                cx[i] = w[(t0 * 16 + y0) * 16 + i] / w0;
+            }
+            bool sep = true;
+            for (int t = 0; t < T && sep; t++) {
+                for (int y = 0; y < 16 && sep; y++) {
+                    for (int x = 0; x < 16; x++) {
+                        if (std::fabs(w[(t * 16 + y) * 16 + x] - at[t] * by[y] * cx[x]) > 1e-9 * peak) {
+                            sep = false;
+                            break;
+                        }
+                    }
+                }
+            }
+            if (sep) {
+                p->separable = 1;
+                for (int x = 0; x < 16; x++) {
+                    for (int n = 0; n < 16; n++) {
+                        const double v = n <= 8 ? std::cos(2 * pi * n * x / 16.0) : std::sin(2 * pi * (n - 8) * x / 16.0);
+                        c[kOffSWX + x * 16 + n] = static_cast<float>(v * cx[x]);
+                        c[kOffSWY + n * 16 + x] = static_cast<float>(v * by[x]);
+                    }
+                }
+                for (int t = 0; t < T; t++) {
+                    c[kOffAS + t] = static_cast<float>(static_cast<double>(scale) * at[t]);
+                }
+                for (int j = 0; j < T; j++) {
+                    const double vr = j <= r ? std::cos(2 * pi * j * r / T) : std::sin(2 * pi * (j - r) * r / T);
+                    c[kOffCTS + j] = static_cast<float>((j == 0 ? 1.0 : 2.0) * vr * at[r]);
                 }
             }
         }
@@ -588,7 +661,8 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
         delete p;
         return fail(DFT2_ERR_ARG, "window_freq[0] must be non-zero when zero_mean is enabled");
     }
-    // Tile of blocks per workgroup: accumulator region at most ~100 samples per side.
+    // Tile of blocks per workgroup (refined by choose_tile once the device is selected):
+    // accumulator region at most ~100 samples per side.
     const int tb = std::min(16, 84 / p->block_step + 1);
     p->tile = make_tile(p, tb, tb);
     if (const char* env = std::getenv("DFT2_SCRATCH_MB")) {
@@ -606,6 +680,7 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
         delete p;
         return rc;
     }
+    p->tile = choose_tile(p);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_consts), sizeof(float) * consts.size());
     if (e == hipSuccess) {
         e = hipMemcpy(p->d_consts, consts.data(), sizeof(float) * consts.size(), hipMemcpyHostToDevice);
@@ -773,16 +848,20 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
         return fail(DFT2_ERR_ARG, "invalid job list");
     }
     // The backward keeps T input-gradient regions per tile in LDS, so it uses smaller
-    // tiles: 8 blocks wide, and 12 tall where that still lets two workgroups share a CU
-    // (measured on 256x256 float clips, T = 3: 8x12 is 8% faster than 8x8; 12x12 and
-    // larger hold one workgroup per CU and are slower).
+    // tiles: 8 blocks wide and as tall as still lets two workgroups share a CU (measured
+    // on 256x256 float clips, T = 3: taller tiles amortise the per-tile work; tiles that
+    // leave one workgroup per CU are slower).
     const int S = p->block_step;
     const int tbw = std::min(8, 84 / S + 1);
     auto region_w_of = [S](int tb) { return ((S * (tb - 1) + 16) + 3) / 4 * 4; };
     auto region_h_of = [S](int tb) { return S * (tb - 1) + 16; };
-    int tbh = std::min(12, 84 / S + 1);
-    if (backward_lds_bytes(p->radius, region_w_of(tbw), region_h_of(tbh)) > 80 * 1024) {
-        tbh = tbw;
+    int tbh = tbw;
+    for (int cand : {16, 12}) {
+        const int h = std::min(cand, 84 / S + 1);
+        if (h > tbh && backward_lds_bytes(p->radius, region_w_of(tbw), region_h_of(h)) <= 80 * 1024) {
+            tbh = h;
+            break;
+        }
     }
     const int region_w = region_w_of(tbw);
     const int region_h = region_h_of(tbh);

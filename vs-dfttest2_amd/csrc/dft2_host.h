@@ -96,6 +96,7 @@ struct dft2_plan {
     int filter_type = 0;
     int zero_mean = 0;
     int asym = 0;
+    int separable = 0; // window = a[t] b[y] c[x] (see kOffSWX)
     int sample_code = dft2::kU8;
     int precision = DFT2_PRECISION_FP32;
     int bits = 8;
@@ -123,13 +124,16 @@ namespace host {
 // Kernel selection for plan p; asym = -1 takes the plan's own sigma symmetry.
 inline FusedKey fused_key(const dft2_plan* p, int persistent = 0, int asym = -1) {
     return FusedKey{p->radius, p->sample_code, p->precision, p->filter_type, persistent,
-                    asym < 0 ? p->asym : asym};
+                    asym < 0 ? p->asym : asym, p->separable && !persistent ? 1 : 0};
 }
 
 // Forward tile of bw x bh blocks for plan p.  Valid when every pixel is covered by the
 // regions of at most 2 tiles per axis (the finalize kernel gathers at most 4).
 TileShape make_tile(const dft2_plan* p, int bw, int bh);
 bool tile_valid(const dft2_plan* p, int bw, int bh);
+// Default forward tile: the candidate with the most blocks in flight per CU (resident
+// workgroups from the occupancy API x blocks per tile); needs the plan's device current.
+TileShape choose_tile(const dft2_plan* p);
 
 // Launch arguments common to every kernel of a plan, for tile `ts`.
 LaunchArgs make_args(const dft2_plan* p, const TileShape& ts);

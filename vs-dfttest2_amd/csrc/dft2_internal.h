@@ -32,7 +32,17 @@ constexpr int kOffWT = kOffSYN + 256;         // up to 49, padded to 64
 constexpr int kOffCT = kOffWT + 64;           // up to 7, padded to 16
 constexpr int kOffTAB = kOffCT + 16;          // up to 4*4*64*4
 constexpr int kTabFloats(int radius) { return (radius + 1) * 4 * 64 * 4; }
-constexpr int kConstFloats = kOffTAB + 4 * 4 * 64 * 4;
+// Separable window w[t][y][x] = a[t] b[y] c[x] (the form DFTTest builds): the spatial
+// factors fold into the bases, so the kernel needs no per-lane window values.
+//   SWX [x][n] = WX[x][n] c[x]    (forward x operand; read transposed: inverse x operand)
+//   SWY [m][y] = WY[m][y] b[y]    (forward y operand; read transposed: inverse y operand)
+//   AS  [t]    = sample scale * a[t]   (per-slice factor of the analysis window)
+//   CTS [j]    = CT[j] * a[r]          (centre weights with the synthesis window's a[r])
+constexpr int kOffSWX = kOffTAB + 4 * 4 * 64 * 4;
+constexpr int kOffSWY = kOffSWX + 256;
+constexpr int kOffAS = kOffSWY + 256;         // up to 7, padded to 8
+constexpr int kOffCTS = kOffAS + 8;           // up to 7, padded to 8
+constexpr int kConstFloats = kOffCTS + 8;
 
 // One plane of one clip in a launch (device copy of dft2_job plus derived geometry).
 struct JobDev {
@@ -93,6 +103,7 @@ struct FusedKey {
     int filter_type;
     int persistent; // 1: persistent grid over the work queue (LaunchArgs::queue)
     int asym;       // LaunchArgs::asym
+    int sep;        // 1: separable-window form (constant table SWX/SWY/AS/CTS); not persistent
 };
 
 // Sample type codes.

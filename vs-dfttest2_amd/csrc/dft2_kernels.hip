@@ -336,7 +336,7 @@ __global__ __launch_bounds__(256) void gain_table_kernel(LaunchArgs a, float4* o
     fill_tab<R, kBias>(out, a, blockIdx.x * 256 + threadIdx.x, gridDim.x * 256);
 }
 
-template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM>
+template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP>
 __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
     constexpr int kWbuf = (T * 256 > 320) ? T * 256 : 320;
@@ -371,23 +371,34 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     fill_tab<R, FTM == 0>(tab, a, tid, kThreads);
 
     // ---- per-lane constant operands ----
+    // SEP: separable window, its spatial factors folded into the bases (kOffSWX), so no
+    // per-lane window values are held (fewer registers, more waves per SIMD).
+    constexpr int kOffX = SEP ? kOffSWX : kOffWX;
+    constexpr int kOffY = SEP ? kOffSWY : kOffWY;
     float wx[4], wy[4], gx[4], gy[4], syn[4];
-    float win[T][4];
+    float win[SEP ? 1 : T][4];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-        wx[s] = a.consts[kOffWX + (4 * lg + s) * 16 + l16];
-        wy[s] = a.consts[kOffWY + l16 * 16 + 4 * lg + s];
+        wx[s] = a.consts[kOffX + (4 * lg + s) * 16 + l16];
+        wy[s] = a.consts[kOffY + l16 * 16 + 4 * lg + s];
         // Inverse operands: the unweighted basis (WX, WY read transposed).  The middle
         // stage leaves every coefficient of a non-degenerate pair at twice its value
         // (the 1/2 of the complex <-> real basis maps is not applied), which is exactly
         // the weight 2 of those basis functions in the inverse.
-        gx[s] = a.consts[kOffWX + l16 * 16 + 4 * lg + s];
-        gy[s] = a.consts[kOffWY + (4 * lg + s) * 16 + l16];
-        syn[s] = a.consts[kOffSYN + (4 * lg + s) * 16 + l16];
+        gx[s] = a.consts[kOffX + l16 * 16 + 4 * lg + s];
+        gy[s] = a.consts[kOffY + (4 * lg + s) * 16 + l16];
+        if constexpr (!SEP) {
+            syn[s] = a.consts[kOffSYN + (4 * lg + s) * 16 + l16];
 #pragma unroll
-        for (int t = 0; t < T; t++) {
-            win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
+            for (int t = 0; t < T; t++) {
+                win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
+            }
         }
+    }
+    float as[T]; // SEP: sample scale * temporal window factor of slice t (uniform)
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        as[t] = SEP ? a.consts[kOffAS + t] : 1.0f;
     }
     const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
     const Opnd WYo = make_opnd<PREC>(wy[0], wy[1], wy[2], wy[3]);
@@ -397,7 +408,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     float ct[T];
 #pragma unroll
     for (int j = 0; j < T; j++) {
-        ct[j] = a.consts[kOffCT + j];
+        ct[j] = a.consts[(SEP ? kOffCTS : kOffCT) + j];
 #pragma unroll
         for (int t = 0; t < T; t++) {
             wt[j][t] = a.consts[kOffWT + j * T + t];
@@ -533,8 +544,16 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const Opnd in = make_opnd<PREC>(win[t][0] * cur[t][0], win[t][1] * cur[t][1], win[t][2] * cur[t][2],
-                                            win[t][3] * cur[t][3]);
+            float wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if constexpr (SEP) {
+                    wv[q] = as[t] * cur[t][q];
+                } else {
+                    wv[q] = win[t][q] * cur[t][q];
+                }
+            }
+            const Opnd in = make_opnd<PREC>(wv[0], wv[1], wv[2], wv[3]);
             const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
             const floatx4 d1 = mm16<PREC>(in, WXo, zero);
             d2[t] = mm16<PREC>(WYo, make_opnd<PREC>(d1[0], d1[1], d1[2], d1[3]), zero);
@@ -674,7 +693,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         float* arow = acc + (bly * S + 4 * lg) * rw + blx * S + l16;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            atomicAdd(arow + i * rw, o[i] * syn[i]);
+            atomicAdd(arow + i * rw, SEP ? o[i] : o[i] * syn[i]);
         }
         wave_lds_sync();
     }
@@ -753,7 +772,6 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     // fraction of the FP32 MFMA cost.
     constexpr int kBP = 1;
     constexpr int kWbuf = T * 256 + 256;
-    constexpr int kSigPad = (T * 144 + 3) / 4 * 4;
     extern __shared__ float smem[];
 
     const int tid = threadIdx.x;
@@ -785,20 +803,20 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     const int rs = sgpr(static_cast<int>(job.src_rs));
     const int go_rs = sgpr(static_cast<int>(job.go_rs));
 
-    // LDS: gain table | sigma gradient | input-gradient regions [T][region_h][region_w] | per-wave staging
+    // LDS: gain table | input-gradient regions [T][region_h][region_w] | per-wave staging.
+    // After the block loop the staging area holds the sigma gradient of the workgroup.
     const int rw = a.region_w;
     const int rows_used = S * (nby - 1) + 16;
     const int cols_used = S * (nbx - 1) + 16;
     const int region = a.region_h * rw;
     float4* tab = reinterpret_cast<float4*>(smem);
-    float* dsig_lds = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
-    float* gacc = dsig_lds + kSigPad;
-    float* wbuf = gacc + T * region + wave * kWbuf;
+    float* gacc = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
+    float* stage = gacc + T * region;
+    float* wbuf = stage + wave * kWbuf;
     float* vbuf = wbuf + T * 256;
+    float* dsig_lds = stage;
+    static_assert(kWaves * kWbuf >= T * 144, "sigma gradient must fit the staging area");
     fill_tab<R, false>(tab, a, tid, kThreads);
-    for (int i = tid; i < T * 144; i += kThreads) {
-        dsig_lds[i] = 0.0f;
-    }
     for (int i = tid; i < T * region; i += kThreads) {
         gacc[i] = 0.0f;
     }
@@ -1135,6 +1153,9 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     }
 
     __syncthreads();
+    for (int i = tid; i < T * 144; i += kThreads) {
+        dsig_lds[i] = 0.0f;
+    }
     // input gradient: padded tile regions -> source samples (mirror, temporal clamp).
     // Region element i = (ry, rx) is stepped without a division; tiles whose region lies
     // inside the plane skip the mirror.
@@ -1174,6 +1195,7 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     }
 
     // sigma gradient: lanes -> LDS (by coefficient) -> global
+    __syncthreads(); // dsig_lds zeroed by every thread
 #pragma unroll
     for (int g = 0; g <= R; g++) {
 #pragma unroll
@@ -1253,25 +1275,32 @@ __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
     store_sample(dst, sum, a);
 }
 
-template <int R, typename Tin, int PREC, bool PERSIST>
+template <int R, typename Tin, int PREC, bool PERSIST, bool SEP>
 const void* fused_ptr_ft(int filter_type, bool asym) {
     // Gains: 0 Wiener, 1 hard threshold (each with a symmetric and an asymmetric form),
     // 2 the generic form for types 2..6.
     if (filter_type == 0) {
-        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 1>)
-                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 0>);
+        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 1, SEP>)
+                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 0, SEP>);
     }
     if (filter_type == 1) {
-        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 1>)
-                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 0>);
+        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 1, SEP>)
+                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 0, SEP>);
     }
-    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, PERSIST, 2>);
+    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, PERSIST, 2, SEP>);
 }
 
 template <int R, typename Tin, bool PERSIST>
 const void* fused_ptr_prec(const FusedKey& k) {
-    return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, PERSIST>(k.filter_type, k.asym != 0)
-                            : fused_ptr_ft<R, Tin, 0, PERSIST>(k.filter_type, k.asym != 0);
+    // The separable-window form exists for the one-item-per-workgroup launch only.
+    if constexpr (!PERSIST) {
+        if (k.sep) {
+            return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0)
+                                    : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0);
+        }
+    }
+    return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, PERSIST, false>(k.filter_type, k.asym != 0)
+                            : fused_ptr_ft<R, Tin, 0, PERSIST, false>(k.filter_type, k.asym != 0);
 }
 
 template <int R, bool PERSIST>
@@ -1315,7 +1344,7 @@ hipError_t allow_lds(const void* kern, size_t lds) {
 
 size_t backward_lds_bytes(int radius, int region_w, int region_h) {
     const int t_len = 2 * radius + 1;
-    return sizeof(float) * (static_cast<size_t>(radius + 1) * 4 * 64 * 4 + (t_len * 144 + 3) / 4 * 4 +
+    return sizeof(float) * (static_cast<size_t>(radius + 1) * 4 * 64 * 4 +
                             static_cast<size_t>(t_len) * region_w * region_h +
                             static_cast<size_t>(kWaves) * (t_len * 256 + 256));
 }
