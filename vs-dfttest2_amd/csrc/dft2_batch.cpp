@@ -123,7 +123,7 @@ int enqueue(dft2_batch* b, int persistent, hipStream_t stream) {
             a.queue = nullptr;
         }
         HIP_TRY(launch_fused(a, key, grid, b->tile.lds, stream));
-        HIP_TRY(launch_finalize(a, p->sample_code, ch.frames, ch.max_pixels, stream));
+        HIP_TRY(launch_finalize(a, p->sample_code, ch.items, stream));
     }
     return DFT2_OK;
 }

@@ -506,7 +506,7 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         if (ev[1]) {
             HIP_TRY(hipEventRecord(ev[1], stream));
         }
-        HIP_TRY(launch_finalize(a, p->sample_code, ch.frames, ch.max_pixels, stream));
+        HIP_TRY(launch_finalize(a, p->sample_code, ch.items, stream));
         if (ev[2]) {
             HIP_TRY(hipEventRecord(ev[2], stream));
         }

@@ -120,8 +120,8 @@ int fused_workgroups_per_cu(const FusedKey& k, size_t lds_bytes);
 hipError_t launch_gain_table(const LaunchArgs& a, int radius, bool bias, float4* out, hipStream_t stream);
 // Zeroes n work-queue counters with a kernel (safe inside graph capture).
 hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream);
-hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,
-                           hipStream_t stream);
+// Finalize over the n_items work items (frame, tile) of a fused launch.
+hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_items, hipStream_t stream);
 hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds_bytes, hipStream_t stream);
 size_t fused_lds_bytes(int radius, int region_w, int region_h);
 size_t backward_lds_bytes(int radius, int region_w, int region_h);

@@ -1224,55 +1224,80 @@ __device__ __forceinline__ void store_sample(Tout* p, float v, const LaunchArgs&
     }
 }
 
-// Sums the tile regions covering each output pixel (at most 2 per axis) in a fixed
-// order and converts to the sample type.
+// One workgroup per work item (frame, tile): the output pixels the tile owns (its S*tb x
+// S*tb corner of the padded plane; the last tile in a row or column owns its whole
+// region) take the tile's own region plus the 16 - S overlapping columns / rows of the
+// left, top and top-left neighbours' regions, summed in a fixed order (top-left, top,
+// left, own), then converted to the sample type.  Scratch region index = work item.
 template <typename Tout>
 __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
-    const long long jf = blockIdx.y;
-    const JobDev& job = a.jobs[find_job<true>(a.jobs, a.n_jobs, jf)];
-    const int f_local = static_cast<int>(jf - job.frame_begin);
-    const int pix = blockIdx.x * 256 + threadIdx.x;
-    if (pix >= job.width * job.height) {
-        return;
-    }
-    const int y = pix / job.width;
-    const int x = pix - y * job.width;
-    const int py = y + job.oy;
-    const int px = x + job.ox;
+    const int item = blockIdx.x;
+    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const int tiles_x = sgpr(job.tiles_x);
+    const int tiles_y = sgpr(job.tiles_y);
+    const int tiles = tiles_x * tiles_y;
+    const int local = item - sgpr(static_cast<int>(job.item_begin));
+    const int f_local = local / tiles;
+    const int tile = local - f_local * tiles;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
     const int S = a.block_step;
     const int th = S * a.tile_bh;
     const int tw = S * a.tile_bw;
-    const int tiles = job.tiles_x * job.tiles_y;
-    const long long region = static_cast<long long>(a.region_h) * a.region_w;
-    const long long base_item = job.item_begin + static_cast<long long>(f_local) * tiles;
-
-    int ty_list[2], tx_list[2];
-    int nty = 0, ntx = 0;
-    const int ty1 = min(py / th, job.tiles_y - 1);
-    for (int ty = max(ty1 - 1, 0); ty <= ty1; ty++) {
-        const int nb = min(a.tile_bh, job.nby - ty * a.tile_bh);
-        if (py >= ty * th && py < ty * th + S * (nb - 1) + 16) {
-            ty_list[nty++] = ty;
+    const int y_lo = ty * th;
+    const int x_lo = tx * tw;
+    const int y_hi = ty == tiles_y - 1 ? y_lo + S * (sgpr(job.nby) - ty * a.tile_bh - 1) + 16 : y_lo + th;
+    const int x_hi = tx == tiles_x - 1 ? x_lo + S * (sgpr(job.nbx) - tx * a.tile_bw - 1) + 16 : x_lo + tw;
+    const int oy = sgpr(job.oy);
+    const int ox = sgpr(job.ox);
+    const int y0 = max(y_lo, oy);
+    const int y1 = min(y_hi, oy + sgpr(job.height));
+    const int x0 = max(x_lo, ox);
+    const int x1 = min(x_hi, ox + sgpr(job.width));
+    if (y0 >= y1 || x0 >= x1) {
+        return;
+    }
+    const int rw = a.region_w;
+    const long long region = static_cast<long long>(a.region_h) * rw;
+    const float* own = a.scratch + static_cast<long long>(item) * region;
+    const float* left = own - region;
+    const float* top = own - tiles_x * region;
+    const float* corner = top - region;
+    const int ov = 16 - S;
+    const bool has_left = tx > 0;
+    const bool has_top = ty > 0;
+    Tout* dst = reinterpret_cast<Tout*>(sgpr_ptr(job.dst)) +
+                static_cast<long long>(sgpr(job.f_begin) + f_local) * sgpr64(job.dst_fp);
+    const long long drs = sgpr64(job.dst_rs);
+    const int w = x1 - x0;
+    const int h = y1 - y0;
+    // pixel i of the w x h area is (r, c) = (i / w, i % w), stepped without a division
+    const int dr = 256 / w;
+    const int dc = 256 - dr * w;
+    int r = threadIdx.x / w;
+    int c = threadIdx.x - r * w;
+    for (; r < h;) {
+        const int ly = y0 + r - y_lo;
+        const int lx = x0 + c - x_lo;
+        float sum = 0.0f;
+        if (has_top && ly < ov) {
+            if (has_left && lx < ov) {
+                sum += corner[(ly + th) * rw + lx + tw];
+            }
+            sum += top[(ly + th) * rw + lx];
+        }
+        if (has_left && lx < ov) {
+            sum += left[ly * rw + lx + tw];
+        }
+        sum += own[ly * rw + lx];
+        store_sample(dst + (y0 + r - oy) * drs + (x0 + c - ox), sum, a);
+        c += dc;
+        r += dr;
+        if (c >= w) {
+            c -= w;
+            ++r;
         }
     }
-    const int tx1 = min(px / tw, job.tiles_x - 1);
-    for (int tx = max(tx1 - 1, 0); tx <= tx1; tx++) {
-        const int nb = min(a.tile_bw, job.nbx - tx * a.tile_bw);
-        if (px >= tx * tw && px < tx * tw + S * (nb - 1) + 16) {
-            tx_list[ntx++] = tx;
-        }
-    }
-    float sum = 0.0f;
-    for (int i = 0; i < nty; i++) {
-        for (int k = 0; k < ntx; k++) {
-            const int ty = ty_list[i], tx = tx_list[k];
-            const long long it = base_item + ty * job.tiles_x + tx;
-            sum += a.scratch[it * region + static_cast<long long>(py - ty * th) * a.region_w + (px - tx * tw)];
-        }
-    }
-    Tout* dst = reinterpret_cast<Tout*>(job.dst) + static_cast<long long>(job.f_begin + f_local) * job.dst_fp +
-                static_cast<long long>(y) * job.dst_rs + x;
-    store_sample(dst, sum, a);
 }
 
 template <int R, typename Tin, int PREC, bool PERSIST, bool SEP>
@@ -1438,9 +1463,8 @@ hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream) {
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_job_frames, int max_pixels,
-                           hipStream_t stream) {
-    dim3 grid(static_cast<unsigned>((max_pixels + 255) / 256), static_cast<unsigned>(n_job_frames));
+hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_items, hipStream_t stream) {
+    const dim3 grid(static_cast<unsigned>(n_items));
     switch (sample_code) {
     case kU8:
         hipLaunchKernelGGL(finalize_kernel<uint8_t>, grid, dim3(256), 0, stream, a);
