@@ -26,11 +26,11 @@ for _p in (ROOT, os.path.join(ROOT, "vs-dfttest2_amd")):
 METRIC = "Mpx/s at 1080p & 4K, plain FP32 vs split-BF16 MFMA at ≤1 LSB; fwd+bwd Mpx/s"
 WIDTH, HEIGHT, FRAMES = 1920, 1080, 240
 PEAK_FP32_MFMA_TFLOPS = 157.3
-# HBM bytes per fused launch on the headline workload (one launch per step), from
-# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by the same counters on a plain
-# copy kernel of known size (profiles/fused_kernel_1080p_yuv420_u8_fp32.md); None until
-# measured for the current kernel.
-FUSED_TRAFFIC_BYTES = None
+# HBM bytes per fused launch on the headline workload (one launch per step): rocprofv3
+# --pmc FETCH_SIZE 3,493,600 KiB x 2 + WRITE_SIZE 4,603,720 KiB x 1, the factors being what
+# the same counters read for a plain copy kernel of known size
+# (profiles/fused_kernel_1080p_yuv420_u8_fp32.md).  Valid for the FP32 default workload.
+FUSED_TRAFFIC_BYTES = (3_493_600 * 2 + 4_603_720) * 1024
 
 
 def flop_per_block(t_len: int) -> int:
