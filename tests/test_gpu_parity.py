@@ -13,7 +13,7 @@ import pytest
 
 from dfttest2_amd import _lib, synth
 from dfttest2_amd.dfttest2 import Backend, VideoClip, filter_clip, get_plan, plane_job
-from dfttest2_amd.params import dfttest_params
+from dfttest2_amd.params import dfttest_params, rdft
 from golden_util import fixture_names, load_fixture, max_err
 from oracle import dfttest_np as onp
 
@@ -266,3 +266,22 @@ def test_repeated_and_alternating_job_tables():
             assert torch.max(torch.abs(ys[i] - refs[i])).item() <= 1e-6, (rep, i)
     plan.close()
     fresh.close()
+
+
+@pytest.mark.parametrize("fmt,precision,asym", [("u8", "fp32", False), ("f32", "fp32", True),
+                                                ("u10", "split_bf16", False)])
+def test_non_separable_window(fmt, precision, asym):
+    # A window that is not a[t] b[y] c[x] takes the kernel form that holds the window per
+    # lane (separable windows, the ones DFTTest builds, fold it into the bases instead).
+    announce("non-separable window", fmt, precision, asym)
+    p = dfttest_params()
+    rng = np.random.default_rng(11)
+    w = np.asarray(p.window, dtype=np.float64).reshape(p.t_len, 16, 16)
+    p.window = (w * rng.uniform(0.7, 1.3, w.shape)).reshape(-1)
+    p.window_freq = np.asarray(rdft([v * 255 for v in p.window], (p.t_len, 16, 16)))
+    if asym:
+        p.sigma = np.abs(rng.normal(1.0, 3.0, p.t_len * 16 * 9)) * 2.0
+    x, bits, is_float = make_plane(fmt, (5, 52, 60), 17)
+    out = gpu_filter(x, p, bits, is_float, Backend.MI355X(precision=precision))
+    bound = tol(is_float) if precision == "fp32" else (5e-5 if is_float else 1.0)
+    assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= bound
