@@ -146,3 +146,26 @@ def test_batch_argument_checks():
     assert empty.info()["work_items"] == 0
     empty.close()
     plan.close()
+
+
+def test_persistent_form_needs_separable_window():
+    # The persistent launch form is built for separable windows only: forcing it on another
+    # window is an argument error, and the autotuner does not offer it.
+    announce("persistent form, non-separable window")
+    p = dfttest_params()
+    w = np.asarray(p.window, dtype=np.float64).reshape(p.t_len, 16, 16)
+    w[1, 3, 5] *= 1.5
+    p.window = w.reshape(-1)
+    from dfttest2_amd.params import rdft
+    p.window_freq = np.asarray(rdft([v * 255 for v in p.window], (p.t_len, 16, 16)))
+    plan = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8)
+    x = torch.from_numpy(synth.int_plane(3, 40, 48, seed=7, bits=8, sd_lsb=6.0).numpy()).cuda()
+    y = torch.empty_like(x)
+    with pytest.raises(_lib.Dft2Error, match="separable"):
+        _lib.Batch(plan, [plane_job(x, y)], autotune=False, persistent=True)
+    batch = _lib.Batch(plan, [plane_job(x, y)], autotune=True, sample_frames=1)
+    assert all(c["persistent"] == 0 for c in batch.candidates())
+    batch.launch()
+    assert max_err(y.cpu().numpy(), onp.process_plane(x.cpu().numpy(), p, 8, False)) <= 1
+    batch.close()
+    plan.close()

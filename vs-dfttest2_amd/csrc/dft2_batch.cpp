@@ -279,6 +279,9 @@ int create(dft2_plan* p, const dft2_job* jobs, int n_jobs, const dft2_batch_opti
             return fail(DFT2_ERR_ARG, "tile %dx%d needs %zu bytes of LDS (at most %zu)", bw, bh, b->tile.lds, kMaxLds);
         }
         b->persistent = opt->persistent ? 1 : 0;
+        if (b->persistent && !p->separable) {
+            return fail(DFT2_ERR_ARG, "the persistent launch form needs a separable window (a[t] b[y] c[x])");
+        }
         b->wg_per_cu = fused_workgroups_per_cu(fused_key(p, b->persistent), b->tile.lds);
         if (b->wg_per_cu <= 0) {
             return fail(DFT2_ERR_ARG, "tile %dx%d does not fit on a CU", bw, bh);

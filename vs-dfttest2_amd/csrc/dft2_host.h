@@ -124,7 +124,7 @@ namespace host {
 // Kernel selection for plan p; asym = -1 takes the plan's own sigma symmetry.
 inline FusedKey fused_key(const dft2_plan* p, int persistent = 0, int asym = -1) {
     return FusedKey{p->radius, p->sample_code, p->precision, p->filter_type, persistent,
-                    asym < 0 ? p->asym : asym, p->separable && !persistent ? 1 : 0};
+                    asym < 0 ? p->asym : asym, p->separable ? 1 : 0};
 }
 
 // Forward tile of bw x bh blocks for plan p.  Valid when every pixel is covered by the

@@ -103,7 +103,8 @@ struct FusedKey {
     int filter_type;
     int persistent; // 1: persistent grid over the work queue (LaunchArgs::queue)
     int asym;       // LaunchArgs::asym
-    int sep;        // 1: separable-window form (constant table SWX/SWY/AS/CTS); not persistent
+    int sep;        // 1: separable-window form (constant table SWX/SWY/AS/CTS); the persistent
+                    // form exists only with it
 };
 
 // Sample type codes.

@@ -1317,15 +1317,22 @@ const void* fused_ptr_ft(int filter_type, bool asym) {
 
 template <int R, typename Tin, bool PERSIST>
 const void* fused_ptr_prec(const FusedKey& k) {
-    // The separable-window form exists for the one-item-per-workgroup launch only.
-    if constexpr (!PERSIST) {
+    // The persistent form exists for separable windows only; the one-item-per-workgroup
+    // form exists for both.
+    if constexpr (PERSIST) {
+        if (!k.sep) {
+            return nullptr;
+        }
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, true, true>(k.filter_type, k.asym != 0)
+                                : fused_ptr_ft<R, Tin, 0, true, true>(k.filter_type, k.asym != 0);
+    } else {
         if (k.sep) {
             return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0)
                                     : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0);
         }
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, false>(k.filter_type, k.asym != 0)
+                                : fused_ptr_ft<R, Tin, 0, false, false>(k.filter_type, k.asym != 0);
     }
-    return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, PERSIST, false>(k.filter_type, k.asym != 0)
-                            : fused_ptr_ft<R, Tin, 0, PERSIST, false>(k.filter_type, k.asym != 0);
 }
 
 template <int R, bool PERSIST>
@@ -1410,6 +1417,9 @@ hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, s
         return hipErrorInvalidValue;
     }
     const void* kern = fused_ptr(k);
+    if (!kern) {
+        return hipErrorInvalidValue;
+    }
     hipError_t e = allow_lds(kern, lds_bytes);
     if (e != hipSuccess) {
         return e;
@@ -1424,7 +1434,7 @@ hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, s
 
 int fused_workgroups_per_cu(const FusedKey& k, size_t lds_bytes) {
     const void* kern = fused_ptr(k);
-    if (allow_lds(kern, lds_bytes) != hipSuccess) {
+    if (!kern || allow_lds(kern, lds_bytes) != hipSuccess) {
         return 0;
     }
     int n = 0;
