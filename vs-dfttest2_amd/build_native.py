@@ -2,13 +2,14 @@
 
     python vs-dfttest2_amd/build_native.py
 
-Compiles csrc/dft2_kernels.hip (device + launch code), csrc/dft2_capi.cpp and
-csrc/dft2_batch.cpp (C ABI) with hipcc and links them into
-vs-dfttest2_amd/dfttest2_amd/_native/.
+Compiles the gfx950 kernels (csrc/dft2_fused.hip once per radius, dft2_backward.hip,
+dft2_kernels.hip) and the C ABI (dft2_capi.cpp, dft2_batch.cpp) with hipcc, in parallel,
+and links them into vs-dfttest2_amd/dfttest2_amd/_native/.
 """
 
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -19,8 +20,14 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "dfttest2_amd", "_native")
 LIB_NAME = "libdfttest2_mi355x.so"
 ARCH = "gfx950"
-# Kernels and launch code; the C ABI (plans, batches).
-SOURCES = ("dft2_kernels.hip", "dft2_capi.cpp", "dft2_batch.cpp")
+# (source, extra flags, object): the fused kernel is compiled once per radius so that its
+# instantiations build in parallel; then the backward, finalize/launch code and C ABI.
+SOURCES = tuple(("dft2_fused.hip", [f"-DDFT2_FUSED_RADIUS={r}"], f"dft2_fused_r{r}.o") for r in range(4)) + (
+    ("dft2_backward.hip", [], "dft2_backward.o"),
+    ("dft2_kernels.hip", [], "dft2_kernels.o"),
+    ("dft2_capi.cpp", [], "dft2_capi.o"),
+    ("dft2_batch.cpp", [], "dft2_batch.o"),
+)
 
 # Packed FP32 instructions are kept out of every gfx950 code object.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
@@ -45,12 +52,16 @@ def build(verbose: bool = True) -> str:
     obj_dir = os.path.join(HERE, "_build")
     os.makedirs(obj_dir, exist_ok=True)
     hc = hipcc()
-    objs = []
-    for src in SOURCES:
-        obj = os.path.join(obj_dir, os.path.splitext(src)[0] + ".o")
-        run([hc, "-c", "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS,
-             os.path.join(CSRC, src), "-o", obj])
+    cmds, objs = [], []
+    for src, extra, obj_name in SOURCES:
+        obj = os.path.join(obj_dir, obj_name)
+        cmds.append([hc, "-c", "-x", "hip", f"--offload-arch={ARCH}", *COMMON, *DEVICE_FLAGS, *extra,
+                     os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    workers = max(1, min(len(cmds), os.cpu_count() or 1, 8))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+        for fut in [pool.submit(run, c) for c in cmds]:
+            fut.result()
     out = os.path.join(OUT_DIR, LIB_NAME)
     tmp = out + ".tmp"
     run([hc, "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])

@@ -1,0 +1,565 @@
+// Backward pass of the differentiable (Wiener) filter and the per-launch gain table.
+
+#include "dft2_device.h"
+
+namespace dft2 {
+namespace {
+
+// The gain table of one launch with sigma taken from a.sigma_dev, built once into global
+// memory (every workgroup then copies it instead of gathering sigma itself).
+template <int R, bool kBias>
+__global__ __launch_bounds__(256) void gain_table_kernel(LaunchArgs a, float4* out) {
+    a.tab_dev = nullptr;
+    fill_tab<R, kBias>(out, a, blockIdx.x * 256 + threadIdx.x, gridDim.x * 256);
+}
+
+// ---------------------------------------------------------------------------------------
+// Backward pass (Wiener gain, float32 planes): gradients of a loss with respect to the
+// input planes and to the sigma array, given the gradient of the output planes.
+// It recomputes each block's spectrum (forward transform + middle stage) instead of
+// storing it, then applies the adjoint of every stage in reverse:
+//   output gradient -> x synthesis window / scale -> adjoint inverse DFT (two MFMA)
+//   -> adjoint middle stage (mean re-add, gain, zero-mean, complex<->real basis maps)
+//   -> adjoint temporal basis -> adjoint forward DFT (two MFMA) -> x analysis window
+//   -> scatter-add into the input gradient through the same mirror / temporal clamp.
+// With sigma taken from a device array the gain is always the average of a coefficient's
+// own gain and its conjugate partner's (kx = 0, 8), so each sigma entry gets its share.
+
+// Wiener gain g = max((psd - s)/(psd + eps), 0) and its derivatives where g > 0:
+// dg/dpsd = (s + eps)/(psd + eps)^2, dg/ds = -1/(psd + eps); both 0 elsewhere.
+__device__ __forceinline__ float3 wiener_d(float psd, float s) {
+    const float r = __builtin_amdgcn_rcpf(psd + 1e-15f);
+    const float g = (psd - s) * r;
+    return g > 0.0f ? make_float3(g, (s + 1e-15f) * r * r, -r) : make_float3(0.0f, 0.0f, 0.0f);
+}
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v += __shfl_xor(v, off);
+    }
+    return v;
+}
+
+// Adjoint of one gain application: x = A - gf*W, y = g(|x|^2) x + gf*W.  (dyr, dyi) in,
+// gradient w.r.t. A out in (dyr, dyi); accumulates d/dsigma (own, partner) and d/dgf.
+__device__ __forceinline__ void gain_adjoint(float ar, float ai, float& dyr, float& dyi, float4 e, float gf,
+                                             float2& dsig, float& dgf) {
+    const float xr = ar - gf * e.z;
+    const float xi = ai - gf * e.w;
+    const float psd = xr * xr + xi * xi;
+    const float3 d1 = wiener_d(psd, e.x);
+    const float3 d2 = wiener_d(psd, e.y);
+    const float g = 0.5f * (d1.x + d2.x);
+    const float h = 0.5f * (d1.y + d2.y);
+    const float sdot = dyr * xr + dyi * xi;
+    const float dxr = g * dyr + 2.0f * h * sdot * xr;
+    const float dxi = g * dyi + 2.0f * h * sdot * xi;
+    dsig.x += 0.5f * sdot * d1.z;
+    dsig.y += 0.5f * sdot * d2.z;
+    dgf += (dyr - dxr) * e.z + (dyi - dxi) * e.w;
+    dyr = dxr;
+    dyi = dxi;
+}
+
+template <int R>
+__global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
+    constexpr int T = 2 * R + 1;
+    // Contractions in split BF16 (hi + lo operands, three products, FP32 accumulation):
+    // about 16 significant bits per operand, well inside the gradients' 1e-3 bound, at a
+    // fraction of the FP32 MFMA cost.
+    constexpr int kBP = 1;
+    constexpr int kWbuf = T * 256 + 256;
+    extern __shared__ float smem[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = sgpr(tid >> 6);
+    const int l16 = lane & 15;
+    const int lg = lane >> 4;
+
+    const int item = blockIdx.x;
+    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const int tiles_x = sgpr(job.tiles_x);
+    const int tiles = tiles_x * sgpr(job.tiles_y);
+    const int local = item - sgpr(static_cast<int>(job.item_begin));
+    const int f_local = local / tiles;
+    const int tile = local - f_local * tiles;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int n = sgpr(job.f_begin) + f_local;
+    const int S = a.block_step;
+    const int by0 = ty * a.tile_bh;
+    const int bx0 = tx * a.tile_bw;
+    const int nby = min(a.tile_bh, sgpr(job.nby) - by0);
+    const int nbx = min(a.tile_bw, sgpr(job.nbx) - bx0);
+    const int nblk = nby * nbx;
+    const int width = sgpr(job.width);
+    const int height = sgpr(job.height);
+    const int ox = sgpr(job.ox);
+    const int oy = sgpr(job.oy);
+    const int rs = sgpr(static_cast<int>(job.src_rs));
+    const int go_rs = sgpr(static_cast<int>(job.go_rs));
+
+    // LDS: gain table | input-gradient regions [T][region_h][region_w] | per-wave staging.
+    // After the block loop the staging area holds the sigma gradient of the workgroup.
+    const int rw = a.region_w;
+    const int rows_used = S * (nby - 1) + 16;
+    const int cols_used = S * (nbx - 1) + 16;
+    const int region = a.region_h * rw;
+    float4* tab = reinterpret_cast<float4*>(smem);
+    float* gacc = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
+    float* stage = gacc + T * region;
+    float* wbuf = stage + wave * kWbuf;
+    float* vbuf = wbuf + T * 256;
+    float* dsig_lds = stage;
+    static_assert(kWaves * kWbuf >= T * 144, "sigma gradient must fit the staging area");
+    fill_tab<R, false>(tab, a, tid, kThreads);
+    for (int i = tid; i < T * region; i += kThreads) {
+        gacc[i] = 0.0f;
+    }
+
+    // The output-gradient chain uses the unweighted inverse basis (the transposes of
+    // which are WX and WY themselves), as the forward does: the adjoint middle stage then
+    // carries no 1/2 factors either.
+    float wx[4], wy[4], wyT[4], wxT[4], synA[4];
+    float win[T][4];
+    const float inv_scale = 1.0f / a.out_scale;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        wx[s] = a.consts[kOffWX + (4 * lg + s) * 16 + l16];
+        wy[s] = a.consts[kOffWY + l16 * 16 + 4 * lg + s];
+        wyT[s] = a.consts[kOffWY + (4 * lg + s) * 16 + l16];
+        wxT[s] = a.consts[kOffWX + l16 * 16 + 4 * lg + s];
+        synA[s] = a.consts[kOffSYN + l16 * 16 + 4 * lg + s] * inv_scale;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
+        }
+    }
+    const Opnd WXo = make_opnd<kBP>(wx[0], wx[1], wx[2], wx[3]);
+    const Opnd WYo = make_opnd<kBP>(wy[0], wy[1], wy[2], wy[3]);
+    const Opnd WYTo = make_opnd<kBP>(wyT[0], wyT[1], wyT[2], wyT[3]);
+    const Opnd WXTo = make_opnd<kBP>(wxT[0], wxT[1], wxT[2], wxT[3]);
+    float wt[T][T];
+    float ct[T];
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+        ct[j] = a.consts[kOffCT + j];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            wt[j][t] = a.consts[kOffWT + j * T + t];
+        }
+    }
+
+    __amdgpu_buffer_rsrc_t frame[T];
+    float* gin[T];
+    {
+        const int n_frames = sgpr(job.n_frames);
+        const float* src = sgpr_ptr(reinterpret_cast<const float*>(job.src));
+        float* gi = sgpr_ptr(reinterpret_cast<float*>(job.grad_in));
+        const long long fp = sgpr64(job.src_fp);
+        const long long gfp = sgpr64(job.gi_fp);
+        const unsigned frame_bytes = static_cast<unsigned>(rs * height) * 4u;
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const int fi = min(max(n - R + t, 0), n_frames - 1);
+            frame[t] = frame_rsrc(src + static_cast<long long>(fi) * fp, frame_bytes);
+            gin[t] = gi + static_cast<long long>(fi) * gfp;
+        }
+    }
+    const __amdgpu_buffer_rsrc_t gout =
+        frame_rsrc(sgpr_ptr(reinterpret_cast<const float*>(job.grad_out)) + static_cast<long long>(n) * sgpr64(job.go_fp),
+                   static_cast<unsigned>(go_rs * height) * 4u);
+    const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg) * 4u;
+    const unsigned lane_off_go = static_cast<unsigned>(l16 * go_rs + 4 * lg) * 4u;
+
+    const int mp = lane >> 3;
+    const int mq = lane & 7;
+    const bool ey = mp != 0;
+    const bool ex = mq != 0;
+    const int wq = l16 & 7;
+    const int wc = l16 >> 3;
+    float2 dsg[R + 1][4];
+#pragma unroll
+    for (int g = 0; g <= R; g++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            dsg[g][k] = make_float2(0.0f, 0.0f);
+        }
+    }
+
+    __syncthreads();
+
+    int bly = wave / nbx;
+    int blx = wave - bly * nbx;
+    for (int b = wave; b < nblk; b += kWaves, blx += kWaves) {
+        while (blx >= nbx) {
+            blx -= nbx;
+            ++bly;
+        }
+        const int y0 = (by0 + bly) * S - oy;
+        const int x0 = (bx0 + blx) * S - ox;
+        const bool inside = y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width;
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+
+        // ---- recompute the forward spectrum, staged as [j][p][q][b*2+c] ----
+        float smp[T][4];
+        if (inside) {
+            const unsigned off = static_cast<unsigned>(y0 * rs + x0) * 4u + lane_off;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    smp[t][s] = raw_value<float>(load_raw<float>(frame[t], off + 4u * s));
+                }
+            }
+        } else {
+            const unsigned roff = static_cast<unsigned>(mirror(y0 + l16, height) * rs);
+            unsigned offs[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                offs[s] = (roff + static_cast<unsigned>(mirror(x0 + 4 * lg + s, width))) * 4u;
+            }
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    smp[t][s] = raw_value<float>(load_raw<float>(frame[t], offs[s]));
+                }
+            }
+        }
+        floatx4 d2[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const Opnd in = make_opnd<kBP>(win[t][0] * smp[t][0], win[t][1] * smp[t][1], win[t][2] * smp[t][2],
+                                         win[t][3] * smp[t][3]);
+            const floatx4 d1 = mm16<kBP>(in, WXo, zero);
+            d2[t] = mm16<kBP>(WYo, make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
+        }
+        {
+            // temporal basis from the sums / differences of slices t and T - t
+            floatx4 tsum[R + 1], tdif[R + 1];
+#pragma unroll
+            for (int t = 1; t <= R; t++) {
+                tsum[t] = d2[t] + d2[T - t];
+                tdif[t] = d2[t] - d2[T - t];
+            }
+#pragma unroll
+            for (int j = 0; j < T; j++) {
+                floatx4 u = d2[0];
+                if (j == 0) {
+#pragma unroll
+                    for (int t = 1; t <= R; t++) {
+                        u += tsum[t];
+                    }
+                } else if (j <= R) {
+#pragma unroll
+                    for (int t = 1; t <= R; t++) {
+                        u += tsum[t] * wt[j][t];
+                    }
+                } else if constexpr (R > 0) {
+                    u = tdif[1] * wt[j][1];
+#pragma unroll
+                    for (int t = 2; t <= R; t++) {
+                        u += tdif[t] * wt[j][t];
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const int m = 4 * lg + i;
+                    wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = u[i];
+                }
+            }
+        }
+
+        // ---- output gradient -> dV = IY^T (dO IX^T), dO = dy / scale * synthesis window ----
+        {
+            float dov[4];
+            if (inside) {
+                const unsigned off = static_cast<unsigned>(y0 * go_rs + x0) * 4u + lane_off_go;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    dov[s] = raw_value<float>(load_raw<float>(gout, off + 4u * s)) * synA[s];
+                }
+            } else {
+                const int fy = y0 + l16;
+                const bool row_ok = fy >= 0 && fy < height;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const int fx = x0 + 4 * lg + s;
+                    const bool ok = row_ok && fx >= 0 && fx < width;
+                    const float v = raw_value<float>(
+                        load_raw<float>(gout, ok ? static_cast<unsigned>(fy * go_rs + fx) * 4u : 0u));
+                    dov[s] = ok ? v * synA[s] : 0.0f;
+                }
+            }
+            const floatx4 pm = mm16<kBP>(make_opnd<kBP>(dov[0], dov[1], dov[2], dov[3]), WXo, zero);
+            const floatx4 dv = mm16<kBP>(WYo, make_opnd<kBP>(pm[0], pm[1], pm[2], pm[3]), zero);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * lg + i;
+                vbuf[((m & 7) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = dv[i];
+            }
+        }
+        wave_lds_sync();
+
+        // ---- adjoint middle stage, one lane per (y pair, x pair) group ----
+        float4 U4[T];
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            U4[j] = *reinterpret_cast<const float4*>(&wbuf[((j * 8 + mp) * 8 + mq) * 4]);
+        }
+        const float4 dV4 = *reinterpret_cast<const float4*>(&vbuf[(mp * 8 + mq) * 4]);
+        const float gf = a.zero_mean ? wbuf[0] * a.inv_wf0 : 0.0f;
+        float dgf = 0.0f;
+        float4 dU4[T];
+#pragma unroll
+        for (int g = 0; g <= R; g++) {
+            const float4 uc = U4[g];
+            const float4 us = g ? U4[R + g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float cc0[2] = {uc.x, uc.y}, cc1[2] = {uc.z, uc.w};
+            const float ss0[2] = {us.x, us.y}, ss1[2] = {us.z, us.w};
+            float z1r[2], z1i[2], z2r[2], z2i[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                z1r[c] = cc0[c] - (ey ? ss1[c] : 0.0f);
+                z1i[c] = -(ss0[c] + (ey ? cc1[c] : 0.0f));
+                z2r[c] = ey ? (cc0[c] + ss1[c]) : cc1[c];
+                z2i[c] = ey ? (cc1[c] - ss0[c]) : -ss1[c];
+            }
+            // forward coefficients A, B of both evaluations (pre zero-mean)
+            float ar[2], ai[2], br[2], bi[2];
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float z0r = e ? z2r[0] : z1r[0];
+                const float z0i = e ? z2i[0] : z1i[0];
+                const float zxr = e ? z2r[1] : z1r[1];
+                const float zxi = e ? z2i[1] : z1i[1];
+                ar[e] = ex ? z0r + zxi : z0r;
+                ai[e] = ex ? z0i - zxr : z0i;
+                br[e] = ex ? z0r - zxi : zxr;
+                bi[e] = ex ? -z0i - zxr : zxi;
+            }
+            // dV -> d(real t,y basis) -> dP, dQ per x slot
+            const float wcg = ct[g];
+            const float wsg = g ? ct[R + g] : 0.0f;
+            const float dv0[2] = {dV4.x, dV4.y}, dv1[2] = {dV4.z, dV4.w};
+            float dzr[2][2], dzi[2][2]; // [e][c] gradients of the gained Z'
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float duc0 = wcg * dv0[c], dus0 = wsg * dv0[c];
+                const float duc1 = wcg * dv1[c], dus1 = wsg * dv1[c];
+                dzr[0][c] = ey ? duc0 - dus1 : duc0;
+                dzr[1][c] = ey ? duc0 + dus1 : duc1;
+                dzi[0][c] = ey ? -(dus0 + duc1) : -dus0;
+                dzi[1][c] = ey ? duc1 - dus0 : -dus1;
+            }
+            float gz1r[2], gz1i[2], gz2r[2], gz2i[2]; // gradients of Z (pre gain), [c]
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float d0r = dzr[e][0], d0i = dzi[e][0], d1r = dzr[e][1], d1i = dzi[e][1];
+                float dar = ex ? d0r + d1i : d0r;
+                float dai = ex ? d0i - d1r : d0i;
+                float dbr = ex ? d0r - d1i : d1r;
+                float dbi = ex ? -d0i - d1r : d1i;
+                gain_adjoint(ar[e], ai[e], dar, dai, tab[(g * 4 + 2 * e) * 64 + lane], gf, dsg[g][2 * e], dgf);
+                gain_adjoint(br[e], bi[e], dbr, dbi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, dsg[g][2 * e + 1],
+                             dgf);
+                const float g0r = ex ? dar + dbr : dar;
+                const float g0i = ex ? dai - dbi : dai;
+                const float g1r = ex ? -dai - dbi : dbr;
+                const float g1i = ex ? dar - dbr : dbi;
+                if (e == 0) {
+                    gz1r[0] = g0r, gz1i[0] = g0i, gz1r[1] = g1r, gz1i[1] = g1i;
+                } else {
+                    gz2r[0] = g0r, gz2i[0] = g0i, gz2r[1] = g1r, gz2i[1] = g1i;
+                }
+            }
+            float dc0[2], dc1[2], ds0[2], ds1[2];
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                dc0[c] = ey ? gz1r[c] + gz2r[c] : gz1r[c];
+                ds1[c] = ey ? gz2r[c] - gz1r[c] : -gz2i[c];
+                ds0[c] = ey ? -gz1i[c] - gz2i[c] : -gz1i[c];
+                dc1[c] = ey ? gz2i[c] - gz1i[c] : gz2r[c];
+            }
+            dU4[g] = make_float4(dc0[0], dc0[1], dc1[0], dc1[1]);
+            if (g) {
+                dU4[R + g] = make_float4(ds0[0], ds0[1], ds1[0], ds1[1]);
+            }
+        }
+        dgf = wave_sum(dgf);
+        if (a.zero_mean && lane == 0) {
+            dU4[0].x += dgf * a.inv_wf0;
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            *reinterpret_cast<float4*>(&wbuf[((j * 8 + mp) * 8 + mq) * 4]) = dU4[j];
+        }
+        wave_lds_sync();
+
+        // ---- adjoint temporal basis and forward DFT, scatter into the input gradient ----
+        floatx4 du[T];
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * lg + i;
+                du[j][i] = wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc];
+            }
+        }
+        // adjoint temporal basis: slice t gets C_t + S_t and slice T - t gets C_t - S_t
+        // (C: cos rows, even in t; S: sin rows, odd in t)
+        floatx4 ddt[T];
+        ddt[0] = du[0];
+#pragma unroll
+        for (int j = 1; j <= R; j++) {
+            ddt[0] += du[j];
+        }
+        if constexpr (R > 0) {
+#pragma unroll
+            for (int t = 1; t <= R; t++) {
+                floatx4 c = du[0];
+#pragma unroll
+                for (int j = 1; j <= R; j++) {
+                    c += du[j] * wt[j][t];
+                }
+                floatx4 sn = du[R + 1] * wt[R + 1][t];
+#pragma unroll
+                for (int k = 2; k <= R; k++) {
+                    sn += du[R + k] * wt[R + k][t];
+                }
+                ddt[t] = c + sn;
+                ddt[T - t] = c - sn;
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const floatx4 dd = ddt[t];
+            // dd (m in registers) used as A operand is dD2^T; R1 = dD2^T WY, db^T = WX R1.
+            const floatx4 r1 = mm16<kBP>(make_opnd<kBP>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
+            const floatx4 dbt = mm16<kBP>(WXTo, make_opnd<kBP>(r1[0], r1[1], r1[2], r1[3]), zero);
+            float* arow = gacc + t * region + (bly * S + l16) * rw + blx * S + 4 * lg;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                atomicAdd(arow + i, win[t][i] * dbt[i]);
+            }
+        }
+        wave_lds_sync();
+    }
+
+    __syncthreads();
+    for (int i = tid; i < T * 144; i += kThreads) {
+        dsig_lds[i] = 0.0f;
+    }
+    // input gradient: padded tile regions -> source samples (mirror, temporal clamp).
+    // Region element i = (ry, rx) is stepped without a division; tiles whose region lies
+    // inside the plane skip the mirror.
+    {
+        const int gy0 = by0 * S - oy;
+        const int gx0 = bx0 * S - ox;
+        const bool inside = gy0 >= 0 && gy0 + rows_used <= height && gx0 >= 0 && gx0 + cols_used <= width;
+        const long long gi_rs = sgpr64(job.gi_rs);
+        const int dy = kThreads / rw;
+        const int dx = kThreads - dy * rw;
+        int ry = tid / rw;
+        int rx = tid - ry * rw;
+        for (int i = tid; i < rows_used * rw; i += kThreads) {
+            if (rx < cols_used) {
+                int fy = gy0 + ry;
+                int fx = gx0 + rx;
+                if (!inside) {
+                    fy = mirror(fy, height);
+                    fx = mirror(fx, width);
+                }
+                const long long off = static_cast<long long>(fy) * gi_rs + fx;
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    const float v = gacc[t * region + i];
+                    if (v != 0.0f) {
+                        atomicAdd(gin[t] + off, v);
+                    }
+                }
+            }
+            rx += dx;
+            ry += dy;
+            if (rx >= rw) {
+                rx -= rw;
+                ++ry;
+            }
+        }
+    }
+
+    // sigma gradient: lanes -> LDS (by coefficient) -> global
+    __syncthreads(); // dsig_lds zeroed by every thread
+#pragma unroll
+    for (int g = 0; g <= R; g++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int3 pos = slot_position(g, k, lane, T);
+            atomicAdd(&dsig_lds[sigma_index(pos)], dsg[g][k].x);
+            atomicAdd(&dsig_lds[sigma_index(partner_position(pos, T))], dsg[g][k].y);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < T * 144; i += kThreads) {
+        const float v = dsig_lds[i];
+        if (v != 0.0f) {
+            atomicAdd(&a.sigma_grad[i], v * a.sigma_scale);
+        }
+    }
+}
+
+} // namespace
+
+size_t backward_lds_bytes(int radius, int region_w, int region_h) {
+    const int t_len = 2 * radius + 1;
+    return sizeof(float) * (static_cast<size_t>(radius + 1) * 4 * 64 * 4 +
+                            static_cast<size_t>(t_len) * region_w * region_h +
+                            static_cast<size_t>(kWaves) * (t_len * 256 + 256));
+}
+
+hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds, hipStream_t stream) {
+    auto run = [&](auto kern) -> hipError_t {
+        if (lds > 65536) {
+            hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+            if (e != hipSuccess) {
+                return e;
+            }
+        }
+        hipLaunchKernelGGL(kern, dim3(static_cast<unsigned>(n_items)), dim3(kThreads), lds, stream, a);
+        return hipGetLastError();
+    };
+    switch (radius) {
+    case 0:
+        return run(backward_kernel<0>);
+    case 1:
+        return run(backward_kernel<1>);
+    case 2:
+        return run(backward_kernel<2>);
+    default:
+        return run(backward_kernel<3>);
+    }
+}
+
+hipError_t launch_gain_table(const LaunchArgs& a, int radius, bool bias, float4* out, hipStream_t stream) {
+    auto run = [&](auto kern) -> hipError_t {
+        hipLaunchKernelGGL(kern, dim3(4), dim3(256), 0, stream, a, out);
+        return hipGetLastError();
+    };
+    switch (radius) {
+    case 0:
+        return bias ? run(gain_table_kernel<0, true>) : run(gain_table_kernel<0, false>);
+    case 1:
+        return bias ? run(gain_table_kernel<1, true>) : run(gain_table_kernel<1, false>);
+    case 2:
+        return bias ? run(gain_table_kernel<2, true>) : run(gain_table_kernel<2, false>);
+    default:
+        return bias ? run(gain_table_kernel<3, true>) : run(gain_table_kernel<3, false>);
+    }
+}
+
+} // namespace dft2

@@ -1,0 +1,308 @@
+// Device helpers shared by the gfx950 kernels: MFMA operand forms, buffer loads of
+// source samples, wave-uniform values, the gain and its table.  Everything here has
+// internal linkage; each kernel translation unit includes it.
+#pragma once
+
+#include <type_traits>
+
+#include "dft2_internal.h"
+
+namespace dft2 {
+namespace {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef short shortx4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ floatx4 mfma4(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// One lane's share of a 16-long contraction operand: the value at K slot (lane/16, s)
+// for s = 0..3, plus its split into BF16 hi + lo parts for the split-precision form.
+struct Opnd {
+    float f[4];
+    bf16x4 hi, lo;
+};
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float floatx2 __attribute__((ext_vector_type(2)));
+
+// BF16 hi part of (a, b) as one packed register, and the FP32 values of its halves (a
+// BF16 is the upper half of an FP32 with the same value).
+__device__ __forceinline__ unsigned pack_hi(float a, float b, float& ha, float& hb) {
+    const unsigned h = __builtin_bit_cast(unsigned, __builtin_convertvector(floatx2{a, b}, bf16x2));
+    ha = __builtin_bit_cast(float, h << 16);
+    hb = __builtin_bit_cast(float, h & 0xffff0000u);
+    return h;
+}
+
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+    return __builtin_bit_cast(unsigned, __builtin_convertvector(floatx2{a, b}, bf16x2));
+}
+
+template <int PREC>
+__device__ __forceinline__ Opnd make_opnd(float a0, float a1, float a2, float a3) {
+    Opnd o;
+    o.f[0] = a0;
+    o.f[1] = a1;
+    o.f[2] = a2;
+    o.f[3] = a3;
+    if constexpr (PREC == 1) {
+        // Packed conversions: two values per instruction for hi and for lo.
+        float h0, h1, h2, h3;
+        const unsigned hi01 = pack_hi(a0, a1, h0, h1);
+        const unsigned hi23 = pack_hi(a2, a3, h2, h3);
+        const unsigned lo01 = pack_bf16(a0 - h0, a1 - h1);
+        const unsigned lo23 = pack_bf16(a2 - h2, a3 - h3);
+        o.hi = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(hi23) << 32) | hi01);
+        o.lo = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(lo23) << 32) | lo01);
+    }
+    return o;
+}
+
+// acc += A * B over one 16-long contraction (4 K slots per lane x 4 lane groups).
+// PREC 0: four FP32 MFMA 16x16x4.  PREC 1: split BF16 with FP32 accumulation,
+// A*B ~ Ahi*Bhi + Alo*Bhi + Ahi*Blo, three BF16 MFMA 16x16x16.  The K slot pairing
+// (lane group, s) is the same in every form, so operands need no reshuffle.
+// The K-packed alternative (one 16x16x32 for [Ahi | Alo] x [Bhi | Bhi] plus one 16x16x16)
+// returned corrupted products on gfx950 when the compiler scheduled a VALU write to a
+// source register of the 16x16x32 right after it; with wait states inserted it was
+// correct, and three 16x16x16 ran at the same speed, so this form is used.
+template <int PREC>
+__device__ __forceinline__ floatx4 mm16(const Opnd& A, const Opnd& B, floatx4 acc) {
+    if constexpr (PREC == 0) {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            acc = mfma4(A.f[s], B.f[s], acc);
+        }
+    } else {
+        const shortx4 ahi = __builtin_bit_cast(shortx4, A.hi);
+        const shortx4 alo = __builtin_bit_cast(shortx4, A.lo);
+        const shortx4 bhi = __builtin_bit_cast(shortx4, B.hi);
+        const shortx4 blo = __builtin_bit_cast(shortx4, B.lo);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Orders this wave's LDS accesses before and after this point, so that values one lane
+// writes can be read by another lane of the same wave afterwards.  The LDS executes a
+// wave's instructions in issue order, so only the compiler has to be kept from moving
+// accesses across the point; nothing waits here (the compiler still waits for a load's
+// data before its first use).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ int mirror(int v, int n) {
+    v = v < 0 ? -v : v;
+    return v >= n ? 2 * (n - 1) - v : v;
+}
+
+// Values that are the same in every lane of the wave, moved to scalar registers (the
+// compiler cannot prove that for values derived from threadIdx.x / 64 or read from the
+// job table, and would otherwise keep them, and everything computed from them, per lane).
+__device__ __forceinline__ int sgpr(int v) {
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
+__device__ __forceinline__ long long sgpr64(long long v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(v));
+    const unsigned hi = __builtin_amdgcn_readfirstlane(static_cast<unsigned>(static_cast<unsigned long long>(v) >> 32));
+    return static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
+template <typename P>
+__device__ __forceinline__ P* sgpr_ptr(P* p) {
+    return reinterpret_cast<P*>(sgpr64(reinterpret_cast<long long>(p)));
+}
+
+// Source samples are read with raw buffer loads: a descriptor per source frame (base,
+// size in bytes; loads past the size return 0 instead of faulting) and a 32-bit byte
+// offset per lane, plus an immediate offset per column.  The raw bits stay in a register
+// until the sample is used, so a prefetch keeps its loads in flight.
+constexpr int kBufferDword3 = 0x00020000; // gfx9-family descriptor word 3 (raw access)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), kBufferDword3);
+}
+
+// Raw sample as loaded: the integer sample itself, or the bits of a float sample.
+template <typename Tin>
+using raw_t = typename std::conditional<sizeof(Tin) == 4, unsigned, Tin>::type;
+
+template <typename Tin>
+__device__ __forceinline__ raw_t<Tin> load_raw(__amdgpu_buffer_rsrc_t r, unsigned off_bytes);
+template <>
+__device__ __forceinline__ uint8_t load_raw<uint8_t>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b8(r, off_bytes, 0, 0);
+}
+template <>
+__device__ __forceinline__ uint16_t load_raw<uint16_t>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, off_bytes, 0, 0);
+}
+template <>
+__device__ __forceinline__ unsigned load_raw<float>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0);
+}
+
+template <typename Tin>
+__device__ __forceinline__ float raw_value(raw_t<Tin> v) {
+    if constexpr (sizeof(Tin) == 4) {
+        return __builtin_bit_cast(float, v);
+    } else {
+        return static_cast<float>(v);
+    }
+}
+
+// Source planes live in device global memory; loading through an address-space-1
+// pointer gives global (not flat) loads.
+template <typename Tin>
+using gptr_t = const __attribute__((address_space(1))) Tin*;
+
+template <typename Tin>
+__device__ __forceinline__ gptr_t<Tin> to_global(const Tin* p) {
+    return (gptr_t<Tin>)(p);
+}
+
+// Gain of one coefficient (cpu_source/kernel.hpp:1826-1890).  FTM 0 and 1 are the
+// Wiener and hard-threshold filters; FTM 2 dispatches on the filter type at run time.
+template <int FTM>
+__device__ __forceinline__ float gain_of(float psd, float s, const LaunchArgs& a) {
+    if constexpr (FTM == 0) {
+        // s arrives as sigma + 1e-15 (gain table of a Wiener plan):
+        // (psd - sigma) / (psd + 1e-15) = 1 - (sigma + 1e-15) / (psd + 1e-15).
+        return fmaxf(fmaf(-s, __builtin_amdgcn_rcpf(psd + 1e-15f), 1.0f), 0.0f);
+    } else if constexpr (FTM == 1) {
+        return psd < s ? 0.0f : 1.0f;
+    } else {
+        switch (a.filter_type) {
+        case 0:
+            return fmaxf((psd - s) / (psd + 1e-15f), 0.0f);
+        case 1:
+            return psd < s ? 0.0f : 1.0f;
+        case 2:
+            return s;
+        case 3:
+            return (psd >= a.pmin && psd <= a.pmax) ? s : a.sigma2;
+        case 4:
+            return s * sqrtf(psd * (a.pmax / ((psd + a.pmin) * (psd + a.pmax) + 1e-15f)));
+        case 5:
+            return powf(fmaxf((psd - s) / (psd + 1e-15f), 0.0f), a.pmin);
+        default:
+            return sqrtf(fmaxf((psd - s) / (psd + 1e-15f), 0.0f));
+        }
+    }
+}
+
+// Zero-mean, gain and mean re-add on one complex coefficient.  e = (sigma, sigma of the
+// conjugate partner, Re WF, Im WF) at this coefficient.  With an asymmetric sigma array
+// the coefficients at kx = 0 and kx = 8 take the mean of their own gain and their
+// conjugate partner's gain, which is what the reference's C2R (it drops Im at those
+// columns) makes of two different gains.
+// One gain-table entry from LDS as a single 128-bit read.  Without the empty asm the
+// symmetric form, which does not use the partner sigma, gets split 32/64-bit reads at a
+// 16-byte lane stride: twice the LDS instructions, with bank conflicts.
+template <int ASYM>
+__device__ __forceinline__ float4 tab_entry(const float4* p) {
+    float4 e = *p;
+    if constexpr (ASYM == 0) {
+        asm volatile("" : "+v"(e.y));
+    }
+    return e;
+}
+
+// ASYM: 0 = symmetric sigma (no partner gains), 1 = asymmetric, 2 = decided at run time
+// from a.asym.  The forms are separate kernel instantiations because the compiler turns
+// a run-time test into selects and computes the partner gains for every plan.
+template <int FTM, int ASYM>
+__device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, float gf, const LaunchArgs& a) {
+    float xr = re - gf * e.z;
+    float xi = im - gf * e.w;
+    float psd = xr * xr + xi * xi;
+    float g = gain_of<FTM>(psd, e.x, a);
+    if constexpr (ASYM != 0) {
+        if ((ASYM == 1 || a.asym) && e.y != e.x) { // only the kx = 0, 8 lanes
+            g = 0.5f * (g + gain_of<FTM>(psd, e.y, a));
+        }
+    }
+    re = xr * g + gf * e.z;
+    im = xi * g + gf * e.w;
+}
+
+// Coefficient (kt, ky, kx) owned by middle-stage lane `lane` at temporal pair g and slot
+// (A of evaluation 0, B of 0, A of 1, B of 1); see the middle stage of fused_kernel.
+__device__ __forceinline__ int3 slot_position(int g, int slot, int lane, int t_len) {
+    const int pp = lane >> 3, q = lane & 7;
+    const int e = slot >> 1, k = slot & 1;
+    const int ky = pp != 0 ? (e == 0 ? pp : (16 - pp) & 15) : (e == 0 ? 0 : 8);
+    if (q != 0) {
+        return k == 0 ? make_int3(g, ky, q) : make_int3((t_len - g) % t_len, (16 - ky) & 15, q);
+    }
+    return make_int3(g, ky, k == 0 ? 0 : 8);
+}
+
+// Conjugate partner whose sigma is averaged with `p` at kx = 0, 8 (itself elsewhere).
+__device__ __forceinline__ int3 partner_position(int3 p, int t_len) {
+    if (p.z == 0 || p.z == 8) {
+        return make_int3((t_len - p.x) % t_len, (16 - p.y) & 15, p.z);
+    }
+    return p;
+}
+
+__device__ __forceinline__ int sigma_index(int3 p) {
+    return (p.x * 16 + p.y) * 9 + p.z;
+}
+
+// Index of the last job whose first work item (or first job-frame) is <= v: the job
+// table is sorted by both.  Jobs without work in a chunk share their start with the
+// next job and are skipped by taking the last match.
+template <bool kByFrame>
+__device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, long long v) {
+    int lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const long long start = kByFrame ? jobs[mid].frame_begin : jobs[mid].item_begin;
+        if (start <= v) {
+            lo = mid;
+        } else {
+            hi = mid - 1;
+        }
+    }
+    return lo;
+}
+
+// Gain table into LDS: the plan's table, with sigma taken from a.sigma_dev when given.
+// kBias: sigma + 1e-15 (the form gain_of<0> takes; the plan's table of a Wiener plan
+// holds it already) instead of sigma.
+template <int R, bool kBias>
+__device__ __forceinline__ void fill_tab(float4* tab, const LaunchArgs& a, int tid, int n_threads) {
+    constexpr int T = 2 * R + 1;
+    if (a.tab_dev) { // built for this launch by gain_table_kernel
+        for (int i = tid; i < (R + 1) * 4 * 64; i += n_threads) {
+            tab[i] = a.tab_dev[i];
+        }
+        return;
+    }
+    const float4* gtab = reinterpret_cast<const float4*>(a.consts + kOffTAB);
+    for (int i = tid; i < (R + 1) * 4 * 64; i += n_threads) {
+        float4 e = gtab[i];
+        if (a.sigma_dev) {
+            const int lane = i & 63, slot = (i >> 6) & 3, g = i >> 8;
+            const int3 pos = slot_position(g, slot, lane, T);
+            e.x = a.sigma_dev[sigma_index(pos)] * a.sigma_scale;
+            e.y = a.sigma_dev[sigma_index(partner_position(pos, T))] * a.sigma_scale;
+            if constexpr (kBias) {
+                e.x += 1e-15f;
+                e.y += 1e-15f;
+            }
+        }
+        tab[i] = e;
+    }
+}
+
+} // namespace
+} // namespace dft2

@@ -1,0 +1,475 @@
+// Fused DFTTest block pipeline for gfx950 (MI355X).
+//
+// One workgroup owns a tile of tile_bh x tile_bw blocks of one output frame of one
+// plane.  Each wave64 takes blocks of the tile in turn and, per block:
+//   1. loads the T 16x16 slices straight from HBM in the MFMA A-operand layout (lane
+//      holds row y = lane%16, columns 4*(lane/16)..+3), applying the mirror padding and
+//      temporal clamp in the address calculation (no padded copy is ever made) and the
+//      analysis window (cpu_source/getframe_impl.cpp:105-156);
+//   2. runs the separable forward real DFT as two chained FP32 MFMA contractions per
+//      slice, D2 = WY * (slice * WX), in a real cos/sin basis (16 real values per axis);
+//      the output of the first MFMA is already the B operand of the second, so no
+//      transpose is needed;
+//   3. forms the temporal real basis in registers, stages the T*256 values in LDS and
+//      regroups them so that each lane owns one (y pair, x pair) group: from its 4*T
+//      values it forms its 2*T complex coefficients X[kt][ky][kx], applies zero-mean,
+//      the gain and the mean re-add (cpu_source/kernel.hpp:1819-1897), maps them back
+//      to the real basis and evaluates the inverse over t at the centre slice;
+//   4. runs the inverse real DFT (C2R convention, cpu_source/kernel.hpp:1978-1982) as
+//      two chained MFMA contractions, multiplies by the synthesis window and overlap-
+//      adds into the tile accumulator in LDS (cpu_source/getframe_impl.cpp:158-175).
+// The finished tile region goes to a scratch buffer; the finalize kernel sums the at
+// most 4 tile regions covering each pixel and converts to the sample type with the
+// reference rounding (cpu_source/getframe_impl.cpp:177-221).
+//
+// The real-basis formulation is exact: a 16-point real transform has 16 real degrees
+// of freedom (Re k = 0..8, Im k = 1..7), so every axis is a 16x16 (spatial) or TxT
+// (temporal) real matrix, and the complex 3-D coefficients are sums/differences of the
+// eight cos/sin products (see the middle stage below).
+
+// Compiled once per radius (DFT2_FUSED_RADIUS = 0..3), each object providing
+// fused_kernel_ptr(FusedKey, RadiusTag<R>) for its radius, so the instantiations build
+// in parallel.
+
+#include "dft2_device.h"
+
+#ifndef DFT2_FUSED_RADIUS
+#error "DFT2_FUSED_RADIUS must be defined (0..3)"
+#endif
+
+namespace dft2 {
+namespace {
+
+template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP>
+__global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
+    constexpr int T = 2 * R + 1;
+    constexpr int kWbuf = (T * 256 > 320) ? T * 256 : 320;
+    extern __shared__ float smem[];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = sgpr(tid >> 6);
+    const int l16 = lane & 15;
+    const int lg = lane >> 4;
+
+    // Work items: one per workgroup, or (PERSIST) taken in turn from the work-queue
+    // counter a.queue by a persistent grid until a.n_items are done.  The two forms are
+    // separate instantiations: the loop costs registers the one-item form does not need.
+    // Work items of one launch number fewer than 2^31 (the host splits larger work), so
+    // item arithmetic stays 32-bit.
+    __shared__ int s_item;
+    int item = blockIdx.x;
+    if constexpr (PERSIST) {
+        if (tid == 0) {
+            s_item = static_cast<int>(atomicAdd(a.queue, 1u));
+        }
+        __syncthreads();
+        item = __builtin_amdgcn_readfirstlane(s_item);
+    }
+
+    const int S = a.block_step;
+    const int rw = a.region_w;
+    float* acc = smem;
+    float4* tab = reinterpret_cast<float4*>(smem + a.region_h * rw);
+    float* wbuf = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64) + wave * kWbuf;
+    fill_tab<R, FTM == 0>(tab, a, tid, kThreads);
+
+    // ---- per-lane constant operands ----
+    // SEP: separable window, its spatial factors folded into the bases (kOffSWX), so no
+    // per-lane window values are held (fewer registers, more waves per SIMD).
+    constexpr int kOffX = SEP ? kOffSWX : kOffWX;
+    constexpr int kOffY = SEP ? kOffSWY : kOffWY;
+    float wx[4], wy[4], gx[4], gy[4], syn[4];
+    float win[SEP ? 1 : T][4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        wx[s] = a.consts[kOffX + (4 * lg + s) * 16 + l16];
+        wy[s] = a.consts[kOffY + l16 * 16 + 4 * lg + s];
+        // Inverse operands: the unweighted basis (WX, WY read transposed).  The middle
+        // stage leaves every coefficient of a non-degenerate pair at twice its value
+        // (the 1/2 of the complex <-> real basis maps is not applied), which is exactly
+        // the weight 2 of those basis functions in the inverse.
+        gx[s] = a.consts[kOffX + l16 * 16 + 4 * lg + s];
+        gy[s] = a.consts[kOffY + (4 * lg + s) * 16 + l16];
+        if constexpr (!SEP) {
+            syn[s] = a.consts[kOffSYN + (4 * lg + s) * 16 + l16];
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+                win[t][s] = a.consts[kOffWIN + (t * 16 + l16) * 16 + 4 * lg + s];
+            }
+        }
+    }
+    float as[T]; // SEP: sample scale * temporal window factor of slice t (uniform)
+#pragma unroll
+    for (int t = 0; t < T; t++) {
+        as[t] = SEP ? a.consts[kOffAS + t] : 1.0f;
+    }
+    const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
+    const Opnd WYo = make_opnd<PREC>(wy[0], wy[1], wy[2], wy[3]);
+    const Opnd GXo = make_opnd<PREC>(gx[0], gx[1], gx[2], gx[3]);
+    const Opnd GYo = make_opnd<PREC>(gy[0], gy[1], gy[2], gy[3]);
+    float wt[T][T];
+    float ct[T];
+#pragma unroll
+    for (int j = 0; j < T; j++) {
+        ct[j] = a.consts[(SEP ? kOffCTS : kOffCT) + j];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            wt[j][t] = a.consts[kOffWT + j * T + t];
+        }
+    }
+
+    // Middle-stage ownership: lane = 8*p + q owns y pair p (rows m = p, p+8 of the real
+    // basis) and x pair q (columns n = q, q+8).  Pair 0 is (cos 0, cos 8) on both axes,
+    // two purely real-axis coefficients; pairs 1..7 are (cos k, sin k).
+    const int mp = lane >> 3;
+    const int mq = lane & 7;
+    const bool ey = mp != 0;
+    const bool ex = mq != 0;
+    const int wq = l16 & 7;
+    const int wc = l16 >> 3;
+
+    while (item < a.n_items) {
+    // ---- work item -> (job, frame, tile) ----
+    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const int tiles_x = sgpr(job.tiles_x);
+    const int tiles = tiles_x * sgpr(job.tiles_y);
+    const int local = item - sgpr(static_cast<int>(job.item_begin));
+    const int f_local = local / tiles;
+    const int tile = local - f_local * tiles;
+    const int ty = tile / tiles_x;
+    const int tx = tile - ty * tiles_x;
+    const int n = sgpr(job.f_begin) + f_local;
+    const int by0 = ty * a.tile_bh;
+    const int bx0 = tx * a.tile_bw;
+    const int nby = min(a.tile_bh, sgpr(job.nby) - by0);
+    const int nbx = min(a.tile_bw, sgpr(job.nbx) - bx0);
+    const int nblk = nby * nbx;
+    const int rows_used = S * (nby - 1) + 16;
+    const int width = sgpr(job.width);
+    const int height = sgpr(job.height);
+    const int ox = sgpr(job.ox);
+    const int oy = sgpr(job.oy);
+    // Row stride in samples; a frame holds fewer than 2^31 samples (checked on the host),
+    // so sample offsets within a frame are 32-bit.
+    const int rs = sgpr(static_cast<int>(job.src_rs));
+    const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg);
+
+    for (int i = tid; i < rows_used * rw; i += kThreads) {
+        acc[i] = 0.0f;
+    }
+
+    // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
+    __amdgpu_buffer_rsrc_t frame[T];
+    {
+        const int n_frames = sgpr(job.n_frames);
+        const bool use_fptr = sgpr(job.use_fptr) != 0;
+        const Tin* src = sgpr_ptr(reinterpret_cast<const Tin*>(job.src));
+        const long long fp = sgpr64(job.src_fp);
+        const unsigned frame_bytes = static_cast<unsigned>(rs * height) * static_cast<unsigned>(sizeof(Tin));
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            const int fi = min(max(n - R + t, 0), n_frames - 1);
+            const Tin* base = use_fptr ? sgpr_ptr(reinterpret_cast<const Tin*>(job.fptr[fi]))
+                                       : src + static_cast<long long>(fi) * fp;
+            frame[t] = frame_rsrc(base, frame_bytes);
+        }
+    }
+    // Samples of block (fy, fx) of the tile for this lane: row y = lane%16, columns
+    // 4*(lane/16)..+3 of every slice, as raw bits.  Blocks that lie inside the plane
+    // (most of them) take one offset per lane; the others apply the mirror padding per
+    // sample.
+    constexpr unsigned kSz = sizeof(Tin);
+    auto fetch = [&](int fy, int fx, raw_t<Tin> (&dst)[T][4]) {
+        const int y0 = (by0 + fy) * S - oy;
+        const int x0 = (bx0 + fx) * S - ox;
+        if (y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width) {
+            const unsigned off = (static_cast<unsigned>(y0 * rs + x0) + lane_off) * kSz;
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    dst[t][s] = load_raw<Tin>(frame[t], off + s * kSz);
+                }
+            }
+        } else {
+            const unsigned roff = static_cast<unsigned>(mirror(y0 + l16, height) * rs);
+            unsigned offs[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                offs[s] = (roff + static_cast<unsigned>(mirror(x0 + 4 * lg + s, width))) * kSz;
+            }
+#pragma unroll
+            for (int t = 0; t < T; t++) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    dst[t][s] = load_raw<Tin>(frame[t], offs[s]);
+                }
+            }
+        }
+    };
+
+    // The next block's samples are fetched one iteration ahead, so their load latency
+    // overlaps the current block's transforms.
+    // Block b of the tile is (b / nbx, b % nbx); the wave's blocks advance by kWaves.
+    auto step_block = [&](int& by, int& bx) {
+        bx += kWaves;
+        while (bx >= nbx) {
+            bx -= nbx;
+            ++by;
+        }
+    };
+    int nby_ = wave / nbx;
+    int nbx_ = wave - nby_ * nbx;
+    raw_t<Tin> nxt[T][4];
+    if (wave < nblk) {
+        fetch(nby_, nbx_, nxt);
+    }
+
+    __syncthreads();
+
+    for (int b = wave; b < nblk; b += kWaves) {
+        const int bly = nby_;
+        const int blx = nbx_;
+        float cur[T][4];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                cur[t][s] = raw_value<Tin>(nxt[t][s]);
+            }
+        }
+        step_block(nby_, nbx_);
+        if (b + kWaves < nblk) {
+            fetch(nby_, nbx_, nxt);
+        }
+
+        // ---- forward: D2_t = WY * (slice_t * WX) ----
+        floatx4 d2[T];
+#pragma unroll
+        for (int t = 0; t < T; t++) {
+            float wv[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                if constexpr (SEP) {
+                    wv[q] = as[t] * cur[t][q];
+                } else {
+                    wv[q] = win[t][q] * cur[t][q];
+                }
+            }
+            const Opnd in = make_opnd<PREC>(wv[0], wv[1], wv[2], wv[3]);
+            const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            const floatx4 d1 = mm16<PREC>(in, WXo, zero);
+            d2[t] = mm16<PREC>(WYo, make_opnd<PREC>(d1[0], d1[1], d1[2], d1[3]), zero);
+        }
+
+        // ---- temporal real basis, staged as [j][p][q][b*2+c] ----
+        // cos(2 pi j t/T) is even and sin odd under t -> T - t, so the rows need only
+        // the sums and differences of the slice pairs (t, T - t).
+        floatx4 tsum[R + 1], tdif[R + 1]; // [t] for t = 1..R
+#pragma unroll
+        for (int t = 1; t <= R; t++) {
+            tsum[t] = d2[t] + d2[T - t];
+            tdif[t] = d2[t] - d2[T - t];
+        }
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            floatx4 u;
+            if (j == 0) {
+                u = d2[0];
+#pragma unroll
+                for (int t = 1; t <= R; t++) {
+                    u += tsum[t];
+                }
+            } else if (j <= R) {
+                u = d2[0];
+#pragma unroll
+                for (int t = 1; t <= R; t++) {
+                    u += tsum[t] * wt[j][t];
+                }
+            } else if constexpr (R > 0) {
+                u = tdif[1] * wt[j][1];
+#pragma unroll
+                for (int t = 2; t <= R; t++) {
+                    u += tdif[t] * wt[j][t];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int m = 4 * lg + i;
+                wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc] = u[i];
+            }
+        }
+        wave_lds_sync();
+
+        // ---- middle stage: complex coefficients, gain, centre-slice inverse over t ----
+        float4 U4[T];
+#pragma unroll
+        for (int j = 0; j < T; j++) {
+            U4[j] = *reinterpret_cast<const float4*>(&wbuf[((j * 8 + mp) * 8 + mq) * 4]);
+        }
+        const float gf = a.zero_mean ? wbuf[0] * a.inv_wf0 : 0.0f;
+        float v00 = 0.0f, v01 = 0.0f, v10 = 0.0f, v11 = 0.0f;
+#pragma unroll
+        for (int g = 0; g <= R; g++) {
+            // c* = cos-t component, s* = sin-t component (zero for g = 0);
+            // index [b][c] = (y slot, x slot).
+            const float4 uc = U4[g];
+            const float4 us = g ? U4[R + g] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            const float c00 = uc.x, c01 = uc.y, c10 = uc.z, c11 = uc.w;
+            const float s00 = us.x, s01 = us.y, s10 = us.z, s11 = us.w;
+
+            // Two (t,y) evaluations per x slot c:
+            //   proper y pair:  e1 = (+kt, +ky), e2 = (+kt, -ky)
+            //   pair 0 (rows cos0, cos8): e1 = row ky=0, e2 = row ky=8
+            float z1r[2], z1i[2], z2r[2], z2i[2];
+            {
+                const float cc0[2] = {c00, c01}, cc1[2] = {c10, c11};
+                const float ss0[2] = {s00, s01}, ss1[2] = {s10, s11};
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    z1r[c] = cc0[c] - (ey ? ss1[c] : 0.0f);
+                    z1i[c] = -(ss0[c] + (ey ? cc1[c] : 0.0f));
+                    z2r[c] = ey ? (cc0[c] + ss1[c]) : cc1[c];
+                    z2i[c] = ey ? (cc1[c] - ss0[c]) : -ss1[c];
+                }
+            }
+
+            float zr[2][2], zi[2][2]; // [e][c] after the gains
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const float z0r = e ? z2r[0] : z1r[0];
+                const float z0i = e ? z2i[0] : z1i[0];
+                const float zxr = e ? z2r[1] : z1r[1];
+                const float zxi = e ? z2i[1] : z1i[1];
+                // proper x pair: A = Z0 - i Z1 (+kx), B = conj(Z0) - i conj(Z1) (conjugate
+                // partner side); pair 0: A = column kx=0, B = column kx=8.
+                float ar = ex ? z0r + zxi : z0r;
+                float ai = ex ? z0i - zxr : z0i;
+                float br = ex ? z0r - zxi : zxr;
+                float bi = ex ? -z0i - zxr : zxi;
+                filter_coef<FTM, ASYM>(ar, ai, tab_entry<ASYM>(&tab[(g * 4 + 2 * e) * 64 + lane]), gf, a);
+                filter_coef<FTM, ASYM>(br, bi, tab_entry<ASYM>(&tab[(g * 4 + 2 * e + 1) * 64 + lane]), gf, a);
+                // x slots back from (A, B), doubled for a proper pair (see gx above).
+                zr[e][0] = ex ? ar + br : ar;
+                zi[e][0] = ex ? ai - bi : ai;
+                zr[e][1] = ex ? -(ai + bi) : br;
+                zi[e][1] = ex ? ar - br : bi;
+            }
+
+            // Back to the real (t, y) basis, then the centre-slice inverse weights.
+            const float wc_ = ct[g];
+            const float ws_ = g ? ct[R + g] : 0.0f;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const float pr = zr[0][c], pi = zi[0][c], qr = zr[1][c], qi = zi[1][c];
+                // doubled for a proper y pair (see gy above)
+                const float u_c0 = ey ? pr + qr : pr;     // cos t, y slot 0
+                const float u_s1 = ey ? qr - pr : -qi;    // sin t, y slot 1
+                const float u_s0 = ey ? -(pi + qi) : -pi; // sin t, y slot 0
+                const float u_c1 = ey ? qi - pi : qr;     // cos t, y slot 1
+                const float y0 = wc_ * u_c0 + ws_ * u_s0;
+                const float y1 = wc_ * u_c1 + ws_ * u_s1;
+                if (c == 0) {
+                    v00 += y0;
+                    v10 += y1;
+                } else {
+                    v01 += y0;
+                    v11 += y1;
+                }
+            }
+        }
+        wave_lds_sync();
+        // V[m][n], row pitch 20 floats, aliased onto the staging buffer.
+        wbuf[mp * 20 + mq] = v00;
+        wbuf[mp * 20 + mq + 8] = v01;
+        wbuf[(mp + 8) * 20 + mq] = v10;
+        wbuf[(mp + 8) * 20 + mq + 8] = v11;
+        wave_lds_sync();
+
+        // ---- inverse: out = GY * (V * GX) ----
+        const float4 v = *reinterpret_cast<const float4*>(&wbuf[l16 * 20 + 4 * lg]);
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        const floatx4 f1 = mm16<PREC>(make_opnd<PREC>(v.x, v.y, v.z, v.w), GXo, zero);
+        const floatx4 o = mm16<PREC>(GYo, make_opnd<PREC>(f1[0], f1[1], f1[2], f1[3]), zero);
+
+        // ---- synthesis window + overlap-add into the tile accumulator ----
+        float* arow = acc + (bly * S + 4 * lg) * rw + blx * S + l16;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            atomicAdd(arow + i * rw, SEP ? o[i] : o[i] * syn[i]);
+        }
+        wave_lds_sync();
+    }
+    __syncthreads();
+
+    float* out = a.scratch + static_cast<long long>(item) * (a.region_h * rw);
+    for (int i = tid; i < rows_used * rw; i += kThreads) {
+        out[i] = acc[i];
+    }
+    if constexpr (!PERSIST) {
+        break;
+    }
+    __syncthreads(); // the accumulator is read out and s_item consumed before reuse
+    if (tid == 0) {
+        s_item = static_cast<int>(atomicAdd(a.queue, 1u));
+    }
+    __syncthreads();
+    item = __builtin_amdgcn_readfirstlane(s_item);
+    }
+}
+
+template <int R, typename Tin, int PREC, bool PERSIST, bool SEP>
+const void* fused_ptr_ft(int filter_type, bool asym) {
+    // Gains: 0 Wiener, 1 hard threshold (each with a symmetric and an asymmetric form),
+    // 2 the generic form for types 2..6.
+    if (filter_type == 0) {
+        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 1, SEP>)
+                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 0, SEP>);
+    }
+    if (filter_type == 1) {
+        return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 1, SEP>)
+                    : reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, PERSIST, 0, SEP>);
+    }
+    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, PERSIST, 2, SEP>);
+}
+
+template <int R, typename Tin, bool PERSIST>
+const void* fused_ptr_prec(const FusedKey& k) {
+    // The persistent form exists for separable windows only; the one-item-per-workgroup
+    // form exists for both.
+    if constexpr (PERSIST) {
+        if (!k.sep) {
+            return nullptr;
+        }
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, true, true>(k.filter_type, k.asym != 0)
+                                : fused_ptr_ft<R, Tin, 0, true, true>(k.filter_type, k.asym != 0);
+    } else {
+        if (k.sep) {
+            return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0)
+                                    : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0);
+        }
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, false>(k.filter_type, k.asym != 0)
+                                : fused_ptr_ft<R, Tin, 0, false, false>(k.filter_type, k.asym != 0);
+    }
+}
+
+template <int R, bool PERSIST>
+const void* fused_ptr_type(const FusedKey& k) {
+    switch (k.sample_code) {
+    case kU8:
+        return fused_ptr_prec<R, uint8_t, PERSIST>(k);
+    case kU16:
+        return fused_ptr_prec<R, uint16_t, PERSIST>(k);
+    default:
+        return fused_ptr_prec<R, float, PERSIST>(k);
+    }
+}
+
+} // namespace
+
+const void* fused_kernel_ptr(const FusedKey& k, RadiusTag<DFT2_FUSED_RADIUS>) {
+    return k.persistent ? fused_ptr_type<DFT2_FUSED_RADIUS, true>(k) : fused_ptr_type<DFT2_FUSED_RADIUS, false>(k);
+}
+
+} // namespace dft2

@@ -110,6 +110,14 @@ struct FusedKey {
 // Sample type codes.
 enum SampleCode { kU8 = 0, kU16 = 1, kF32 = 2 };
 
+// Fused kernel instantiations of one radius (each radius is its own object file).
+template <int R>
+struct RadiusTag {};
+const void* fused_kernel_ptr(const FusedKey& k, RadiusTag<0>);
+const void* fused_kernel_ptr(const FusedKey& k, RadiusTag<1>);
+const void* fused_kernel_ptr(const FusedKey& k, RadiusTag<2>);
+const void* fused_kernel_ptr(const FusedKey& k, RadiusTag<3>);
+
 // Fused kernel over `grid` workgroups: one work item each, or a persistent grid over the
 // work queue when a.queue is set.
 hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, size_t lds_bytes,
