@@ -55,6 +55,11 @@ extern "C" {
 #define DFT2_PRECISION_FP32 0      /* FP32 MFMA */
 #define DFT2_PRECISION_SPLIT_BF16 1 /* hi+lo BF16 operands, three products, FP32 accumulate */
 
+/* Stream argument naming the null (default) stream: the work is queued there and not
+ * waited for (NULL instead selects the plan's own stream and a blocking call).  Same
+ * value as HIP's hipStreamLegacy; the library passes stream 0 to HIP for it. */
+#define DFT2_DEFAULT_STREAM ((void*)1)
+
 /* Plugin parameters, with the meaning and layout of the reference DFTTest arguments. */
 typedef struct dft2_params {
     int radius;           /* "radius": 0..3, T = 2*radius+1 */
@@ -119,6 +124,11 @@ DFT2_API int dft2_plan_create(const dft2_params* params, int sample_type, int bi
 
 /* Releases the plan and its device memory. NULL is accepted. */
 DFT2_API int dft2_plan_destroy(dft2_plan* plan);
+
+/* Streams: every `void* stream` below is a hipStream_t.  NULL selects the plan's (or
+ * batch's) own stream and makes the call wait for completion.  Any other handle queues
+ * the work on that stream and returns at once; DFT2_DEFAULT_STREAM queues it on the
+ * null (default) stream, which NULL cannot name here. */
 
 /* Filters one output frame of one plane.  src_frames holds the T = 2*radius+1 source
  * planes in temporal order (already clamped by the caller, as GetFrame gathers them).

@@ -91,3 +91,14 @@ def test_plan_create_rejects_formats():
         _lib.Plan(_params(), _lib.DFT2_SAMPLE_FLOAT, 16)
     with pytest.raises(_lib.Dft2Error, match="num_streams"):
         _lib.Plan(_params(), _lib.DFT2_SAMPLE_INT, 8, num_streams=0)
+
+
+def test_stream_arguments():
+    # None selects the plan's own stream (a blocking call); a torch stream handle is passed
+    # through, with 0 (torch's default stream) as DFT2_DEFAULT_STREAM, which the library
+    # turns into HIP's stream 0 without waiting.
+    assert _lib._stream_arg(None).value is None
+    assert _lib._stream_arg(0).value == _lib.DFT2_DEFAULT_STREAM == 1
+    text = open(HEADER).read()
+    assert re.search(r"#define DFT2_DEFAULT_STREAM \(\(void\*\)1\)", text)
+    assert _lib._stream_arg(0x7F00_1234).value == 0x7F00_1234

@@ -25,14 +25,6 @@ __global__ __launch_bounds__(256) void gain_table_kernel(LaunchArgs a, float4* o
 // With sigma taken from a device array the gain is always the average of a coefficient's
 // own gain and its conjugate partner's (kx = 0, 8), so each sigma entry gets its share.
 
-// Wiener gain g = max((psd - s)/(psd + eps), 0) and its derivatives where g > 0:
-// dg/dpsd = (s + eps)/(psd + eps)^2, dg/ds = -1/(psd + eps); both 0 elsewhere.
-__device__ __forceinline__ float3 wiener_d(float psd, float s) {
-    const float r = __builtin_amdgcn_rcpf(psd + 1e-15f);
-    const float g = (psd - s) * r;
-    return g > 0.0f ? make_float3(g, (s + 1e-15f) * r * r, -r) : make_float3(0.0f, 0.0f, 0.0f);
-}
-
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -41,29 +33,41 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 
-// Adjoint of one gain application: x = A - gf*W, y = g(|x|^2) x + gf*W.  (dyr, dyi) in,
-// gradient w.r.t. A out in (dyr, dyi); accumulates d/dsigma (own, partner) and d/dgf.
+// Adjoint of one gain application: x = A - gf*W, y = g x + gf*W, where the gain is the
+// average of the Wiener gains of the coefficient's own sigma and its partner's,
+// g = (g1 + g2)/2, g_i = max(1 - c_i, 0), c_i = se_i/(psd + eps), se_i = sigma_i + eps
+// (table entries e.x, e.y), psd = |x|^2.  Where g_i > 0, dg_i/dpsd = c_i/(psd + eps) and
+// dg_i/dsigma_i = -1/(psd + eps); both are 0 elsewhere.
+// (dyr, dyi) in: HALF the gradient w.r.t. y (the factor 1/2 of the average is folded into
+// the caller's inverse weights); out: the full gradient w.r.t. A.  Accumulates d/dsigma
+// (own, partner) and d/dgf.
 __device__ __forceinline__ void gain_adjoint(float ar, float ai, float& dyr, float& dyi, float4 e, float gf,
                                              float2& dsig, float& dgf) {
     const float xr = ar - gf * e.z;
     const float xi = ai - gf * e.w;
     const float psd = xr * xr + xi * xi;
-    const float3 d1 = wiener_d(psd, e.x);
-    const float3 d2 = wiener_d(psd, e.y);
-    const float g = 0.5f * (d1.x + d2.x);
-    const float h = 0.5f * (d1.y + d2.y);
+    const float r = __builtin_amdgcn_rcpf(psd + 1e-15f);
+    const float c1 = e.x * r;
+    const float c2 = e.y * r;
+    const bool m1 = c1 < 1.0f; // g1 > 0
+    const bool m2 = c2 < 1.0f;
+    const float g2x = (m1 ? 1.0f - c1 : 0.0f) + (m2 ? 1.0f - c2 : 0.0f); // 2 g
+    const float q = (m1 ? c1 : 0.0f) + (m2 ? c2 : 0.0f);                 // 2 (psd + eps) dg/dpsd
     const float sdot = dyr * xr + dyi * xi;
-    const float dxr = g * dyr + 2.0f * h * sdot * xr;
-    const float dxi = g * dyi + 2.0f * h * sdot * xi;
-    dsig.x += 0.5f * sdot * d1.z;
-    dsig.y += 0.5f * sdot * d2.z;
-    dgf += (dyr - dxr) * e.z + (dyi - dxi) * e.w;
+    const float rs = r * sdot;
+    // dx = g dy + 2 (dg/dpsd) (x . dy) x, with dy = 2 (dyr, dyi)
+    const float k = 2.0f * (q * rs);
+    const float dxr = g2x * dyr + k * xr;
+    const float dxi = g2x * dyi + k * xi;
+    dsig.x = m1 ? dsig.x - rs : dsig.x;
+    dsig.y = m2 ? dsig.y - rs : dsig.y;
+    dgf += (2.0f * dyr - dxr) * e.z + (2.0f * dyi - dxi) * e.w;
     dyr = dxr;
     dyi = dxi;
 }
 
 template <int R>
-__global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1 ? 4 : 1))) void backward_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
     // Contractions in split BF16 (hi + lo operands, three products, FP32 accumulation):
     // about 16 significant bits per operand, well inside the gradients' 1e-3 bound, at a
@@ -93,7 +97,6 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     const int bx0 = tx * a.tile_bw;
     const int nby = min(a.tile_bh, sgpr(job.nby) - by0);
     const int nbx = min(a.tile_bw, sgpr(job.nbx) - bx0);
-    const int nblk = nby * nbx;
     const int width = sgpr(job.width);
     const int height = sgpr(job.height);
     const int ox = sgpr(job.ox);
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     float* vbuf = wbuf + T * 256;
     float* dsig_lds = stage;
     static_assert(kWaves * kWbuf >= T * 144, "sigma gradient must fit the staging area");
-    fill_tab<R, false>(tab, a, tid, kThreads);
+    fill_tab<R, true>(tab, a, tid, kThreads);
     for (int i = tid; i < T * region; i += kThreads) {
         gacc[i] = 0.0f;
     }
@@ -145,7 +148,8 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
     float ct[T];
 #pragma unroll
     for (int j = 0; j < T; j++) {
-        ct[j] = a.consts[kOffCT + j];
+        // halved: gain_adjoint takes half the gradient of the gained coefficients
+        ct[j] = 0.5f * a.consts[kOffCT + j];
 #pragma unroll
         for (int t = 0; t < T; t++) {
             wt[j][t] = a.consts[kOffWT + j * T + t];
@@ -189,31 +193,29 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         }
     }
 
-    __syncthreads();
-
-    int bly = wave / nbx;
-    int blx = wave - bly * nbx;
-    for (int b = wave; b < nblk; b += kWaves, blx += kWaves) {
-        while (blx >= nbx) {
-            blx -= nbx;
-            ++bly;
-        }
-        const int y0 = (by0 + bly) * S - oy;
-        const int x0 = (bx0 + blx) * S - ox;
-        const bool inside = y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width;
-        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-
-        // ---- recompute the forward spectrum, staged as [j][p][q][b*2+c] ----
-        float smp[T][4];
-        if (inside) {
+    // Samples of the T slices and of the output gradient for block (fy, fx) of the tile,
+    // as raw bits, fetched one block ahead (once the current block's samples are
+    // consumed) so that their latency overlaps the current block's work.  Output-gradient samples outside the plane count as 0: their validity
+    // is kept as a 4-bit mask (bit s for column 4*(lane/16) + s) and applied at use.
+    auto fetch = [&](int fy, int fx, unsigned (&dst)[T][4], unsigned (&dgo)[4], unsigned& okm) {
+        fx = opaque_uniform(fx); // a wave's column is fixed: keep its offsets out of the loop's registers
+        const int y0 = (by0 + fy) * S - oy;
+        const int x0 = (bx0 + fx) * S - ox;
+        if (y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width) {
             const unsigned off = static_cast<unsigned>(y0 * rs + x0) * 4u + lane_off;
 #pragma unroll
             for (int t = 0; t < T; t++) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    smp[t][s] = raw_value<float>(load_raw<float>(frame[t], off + 4u * s));
+                    dst[t][s] = load_raw<float>(frame[t], off + 4u * s);
                 }
             }
+            const unsigned og = static_cast<unsigned>(y0 * go_rs + x0) * 4u + lane_off_go;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                dgo[s] = load_raw<float>(gout, og + 4u * s);
+            }
+            okm = 0xfu;
         } else {
             const unsigned roff = static_cast<unsigned>(mirror(y0 + l16, height) * rs);
             unsigned offs[4];
@@ -225,15 +227,46 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
             for (int t = 0; t < T; t++) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    smp[t][s] = raw_value<float>(load_raw<float>(frame[t], offs[s]));
+                    dst[t][s] = load_raw<float>(frame[t], offs[s]);
                 }
             }
+            const int gy = y0 + l16;
+            const bool row_ok = gy >= 0 && gy < height;
+            okm = 0u;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                const int gx = x0 + 4 * lg + s;
+                const bool ok = row_ok && gx >= 0 && gx < width;
+                dgo[s] = load_raw<float>(gout, ok ? static_cast<unsigned>(gy * go_rs + gx) * 4u : 0u);
+                okm |= ok ? (1u << s) : 0u;
+            }
         }
+    };
+    // Each block row r of the tile is done in passes of kWaves blocks: in pass p wave w
+    // takes block (r, p*kWaves + w).  The wave's next block is fetched one block ahead.
+    const int n_pass = (nbx + kWaves - 1) / kWaves;
+    const int my_pass = nbx > wave ? (nbx - wave + kWaves - 1) / kWaves : 0; // passes with a block of this wave
+    unsigned nxt[T][4], nxt_go[4], nxt_ok = 0u;
+    if (my_pass > 0) {
+        fetch(0, wave, nxt, nxt_go, nxt_ok);
+    }
+
+    __syncthreads();
+
+    for (int r = 0; r < nby; r++) {
+    for (int pass = 0; pass < n_pass; pass++) {
+    if (pass < my_pass) {
+        const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+        // ---- recompute the forward spectrum, staged as [j][p][q][b*2+c] ----
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            const Opnd in = make_opnd<kBP>(win[t][0] * smp[t][0], win[t][1] * smp[t][1], win[t][2] * smp[t][2],
-                                         win[t][3] * smp[t][3]);
+            float wv[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                wv[s] = win[t][s] * raw_value<float>(nxt[t][s]);
+            }
+            const Opnd in = make_opnd<kBP>(wv[0], wv[1], wv[2], wv[3]);
             const floatx4 d1 = mm16<kBP>(in, WXo, zero);
             d2[t] = mm16<kBP>(WYo, make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
@@ -276,23 +309,15 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
         // ---- output gradient -> dV = IY^T (dO IX^T), dO = dy / scale * synthesis window ----
         {
             float dov[4];
-            if (inside) {
-                const unsigned off = static_cast<unsigned>(y0 * go_rs + x0) * 4u + lane_off_go;
 #pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    dov[s] = raw_value<float>(load_raw<float>(gout, off + 4u * s)) * synA[s];
-                }
-            } else {
-                const int fy = y0 + l16;
-                const bool row_ok = fy >= 0 && fy < height;
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    const int fx = x0 + 4 * lg + s;
-                    const bool ok = row_ok && fx >= 0 && fx < width;
-                    const float v = raw_value<float>(
-                        load_raw<float>(gout, ok ? static_cast<unsigned>(fy * go_rs + fx) * 4u : 0u));
-                    dov[s] = ok ? v * synA[s] : 0.0f;
-                }
+            for (int s = 0; s < 4; s++) {
+                dov[s] = ((nxt_ok >> s) & 1u) ? raw_value<float>(nxt_go[s]) * synA[s] : 0.0f;
+            }
+            // The samples of this block are consumed: fetch the wave's next block into the
+            // same registers; the loads have the rest of this block's work to complete.
+            const bool row_end = pass + 1 == my_pass;
+            if (!row_end || r + 1 < nby) {
+                fetch(row_end ? r + 1 : r, (row_end ? 0 : pass + 1) * kWaves + wave, nxt, nxt_go, nxt_ok);
             }
             const floatx4 pm = mm16<kBP>(make_opnd<kBP>(dov[0], dov[1], dov[2], dov[3]), WXo, zero);
             const floatx4 dv = mm16<kBP>(WYo, make_opnd<kBP>(pm[0], pm[1], pm[2], pm[3]), zero);
@@ -435,19 +460,53 @@ __global__ __launch_bounds__(kThreads) void backward_kernel(LaunchArgs a) {
                 ddt[T - t] = c - sn;
             }
         }
+        wave_lds_sync(); // du read before the staged outputs overwrite the buffer
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const floatx4 dd = ddt[t];
             // dd (m in registers) used as A operand is dD2^T; R1 = dD2^T WY, db^T = WX R1.
             const floatx4 r1 = mm16<kBP>(make_opnd<kBP>(dd[0], dd[1], dd[2], dd[3]), WYTo, zero);
             const floatx4 dbt = mm16<kBP>(WXTo, make_opnd<kBP>(r1[0], r1[1], r1[2], r1[3]), zero);
-            float* arow = gacc + t * region + (bly * S + l16) * rw + blx * S + 4 * lg;
+            // slice t's block gradient (row l16, columns 4*(lane/16)..+3) staged as [y][x]
+            // at offset t*256 + wave*S of the wave's buffer
+            float* srow = wbuf + t * 256 + wave * S + l16 * 16 + 4 * lg;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                atomicAdd(arow + i, win[t][i] * dbt[i]);
+                srow[i] = win[t][i] * dbt[i];
             }
         }
-        wave_lds_sync();
+    }
+        // ---- overlap-add of pass `pass` of block row r into the T gradient regions ----
+        // As in the forward kernel: each pixel of the pass's 16-row band sums the staged
+        // values of the pass's blocks covering it (block k's value for local column x at
+        // k*kWbuf + t*256 + y*16 + x) and its owning thread adds the sum, no atomics.
+        __syncthreads();
+        {
+            const int Sg = opaque_uniform(S); // per-lane bounds stay in the loop
+            const int nb = min(kWaves, nbx - pass * kWaves);
+            const int cols = Sg * (nb - 1) + 16;
+#pragma unroll
+            for (int yy = 0; yy < 16; yy += kWaves) {
+                const int y = yy + wave;
+                for (int x = lane; x < cols; x += 64) {
+                    // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)
+                    const int lo = x >= 15 ? (x - 15 + Sg - 1) / Sg : 0;
+                    const int hi = min(x / Sg, nb - 1);
+                    const int g = (r * Sg + y) * rw + pass * kWaves * Sg + x;
+#pragma unroll
+                    for (int t = 0; t < T; t++) {
+                        const float* sy = stage + t * 256 + y * 16 + x;
+                        float sum = 0.0f;
+                        for (int k = lo; k <= hi; k++) {
+                            sum += sy[k * kWbuf];
+                        }
+                        gacc[t * region + g] += sum;
+                    }
+                }
+            }
+        }
+        __syncthreads(); // staged values consumed before the next pass reuses the buffers
+    }
     }
 
     __syncthreads();

@@ -359,7 +359,7 @@ int dft2_batch_launch(dft2_batch* b, void* stream) {
     if (!b->exec) {
         return DFT2_OK; // nothing to do
     }
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : b->stream;
+    hipStream_t st = call_stream(stream, b->stream);
     HIP_TRY(hipGraphLaunch(b->exec, st));
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));

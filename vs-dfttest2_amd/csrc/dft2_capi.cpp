@@ -62,7 +62,9 @@ bool tile_valid(const dft2_plan* p, int bw, int bh) {
 }
 
 TileShape choose_tile(const dft2_plan* p) {
-    static const int kCand[][2] = {{16, 16}, {12, 16}, {16, 12}, {12, 12}, {8, 16}, {16, 8}, {8, 12}, {8, 8}};
+    // A block row of a tile is computed in passes of kWaves blocks (one per wave), so
+    // widths that are multiples of kWaves keep every wave busy.
+    static const int kCand[][2] = {{8, 24}, {16, 12}, {8, 16}, {16, 16}, {16, 8}, {8, 12}, {16, 24}, {8, 8}};
     const int tb_max = 84 / p->block_step + 1; // accumulator region at most ~100 samples per side
     TileShape best = make_tile(p, std::min(16, tb_max), std::min(16, tb_max));
     long long best_score = -1;
@@ -73,7 +75,9 @@ TileShape choose_tile(const dft2_plan* p) {
         }
         const TileShape ts = make_tile(p, bw, bh);
         const int wg = fused_workgroups_per_cu(fused_key(p), ts.lds);
-        const long long score = static_cast<long long>(wg) * bw * bh;
+        const int waves_used = kWaves * ((bw + kWaves - 1) / kWaves); // wave slots of one block row
+        // blocks per CU in flight, discounted by the idle wave slots of partial passes
+        const long long score = static_cast<long long>(wg) * bh * bw * bw * 64 / waves_used;
         if (wg > 0 && score > best_score) {
             best_score = score;
             best = ts;
@@ -820,7 +824,7 @@ int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, con
     }
     Slot* slot = acquire_slot(p);
     SlotGuard guard{p, slot};
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : slot->stream;
+    hipStream_t st = call_stream(stream, slot->stream);
     if (stream) {
         HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     }
@@ -912,7 +916,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     }
     Slot* slot = acquire_slot(p);
     SlotGuard guard{p, slot};
-    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : slot->stream;
+    hipStream_t st = call_stream(stream, slot->stream);
     if (stream) {
         HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     }
@@ -932,7 +936,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     a.tile_bh = tbh;
     a.region_w = region_w;
     a.region_h = region_h;
-    rc = build_gain_table(p, slot, a, false, st);
+    rc = build_gain_table(p, slot, a, true, st); // sigma + 1e-15, as gain_adjoint takes it
     if (rc) {
         return rc;
     }
@@ -985,7 +989,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         j.dst_fp = 0;
         Slot* slot = acquire_slot(p);
         SlotGuard guard{p, slot};
-        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : slot->stream;
+        hipStream_t st = call_stream(stream, slot->stream);
         if (stream) {
             HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
         }

@@ -97,6 +97,14 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_wave_barrier();
 }
 
+// Returns a wave-uniform value the optimizer cannot see through, so that per-lane values
+// derived from it inside a loop are recomputed there rather than hoisted out of the loop
+// into registers that stay live across every iteration.
+__device__ __forceinline__ int opaque_uniform(int v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
+
 __device__ __forceinline__ int mirror(int v, int n) {
     v = v < 0 ? -v : v;
     return v >= n ? 2 * (n - 1) - v : v;

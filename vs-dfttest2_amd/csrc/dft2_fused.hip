@@ -40,10 +40,16 @@
 namespace dft2 {
 namespace {
 
+// Radius 0/1, Wiener or threshold gain, separable window: at most 80 VGPRs, so that six
+// waves per SIMD (three workgroups per CU) fit.
 template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP>
-__global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
+__global__ __launch_bounds__(kThreads)
+__attribute__((amdgpu_waves_per_eu(R <= 1 && FTM != 2 && SEP && !PERSIST ? 6 : 1)))
+void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
-    constexpr int kWbuf = (T * 256 > 320) ? T * 256 : 320;
+    // Per-wave LDS: the T*256 spectrum staging, reused for V (16 x 20) and for the block
+    // output staged at offset wave*S (<= 7*16 floats) for the overlap-add.
+    constexpr int kWbuf = (T * 256 > 384) ? T * 256 : 384;
     extern __shared__ float smem[];
 
     const int tid = threadIdx.x;
@@ -71,7 +77,8 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     const int rw = a.region_w;
     float* acc = smem;
     float4* tab = reinterpret_cast<float4*>(smem + a.region_h * rw);
-    float* wbuf = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64) + wave * kWbuf;
+    float* stage = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
+    float* wbuf = stage + wave * kWbuf;
     fill_tab<R, FTM == 0>(tab, a, tid, kThreads);
 
     // ---- per-lane constant operands ----
@@ -144,7 +151,6 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     const int bx0 = tx * a.tile_bw;
     const int nby = min(a.tile_bh, sgpr(job.nby) - by0);
     const int nbx = min(a.tile_bw, sgpr(job.nbx) - bx0);
-    const int nblk = nby * nbx;
     const int rows_used = S * (nby - 1) + 16;
     const int width = sgpr(job.width);
     const int height = sgpr(job.height);
@@ -181,6 +187,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
     // sample.
     constexpr unsigned kSz = sizeof(Tin);
     auto fetch = [&](int fy, int fx, raw_t<Tin> (&dst)[T][4]) {
+        fx = opaque_uniform(fx); // a wave's column is fixed: keep its offsets out of the loop's registers
         const int y0 = (by0 + fy) * S - oy;
         const int x0 = (bx0 + fx) * S - ox;
         if (y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width) {
@@ -209,28 +216,21 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         }
     };
 
-    // The next block's samples are fetched one iteration ahead, so their load latency
-    // overlaps the current block's transforms.
-    // Block b of the tile is (b / nbx, b % nbx); the wave's blocks advance by kWaves.
-    auto step_block = [&](int& by, int& bx) {
-        bx += kWaves;
-        while (bx >= nbx) {
-            bx -= nbx;
-            ++by;
-        }
-    };
-    int nby_ = wave / nbx;
-    int nbx_ = wave - nby_ * nbx;
+    // Each block row r of the tile is done in passes of kWaves blocks: in pass p wave w
+    // computes block (r, p*kWaves + w).  The wave's next block is fetched one block ahead,
+    // so its load latency overlaps the current block's transforms.
+    const int n_pass = (nbx + kWaves - 1) / kWaves;
+    const int my_pass = nbx > wave ? (nbx - wave + kWaves - 1) / kWaves : 0; // passes with a block of this wave
     raw_t<Tin> nxt[T][4];
-    if (wave < nblk) {
-        fetch(nby_, nbx_, nxt);
+    if (my_pass > 0) {
+        fetch(0, wave, nxt);
     }
 
     __syncthreads();
 
-    for (int b = wave; b < nblk; b += kWaves) {
-        const int bly = nby_;
-        const int blx = nbx_;
+    for (int r = 0; r < nby; r++) {
+    for (int pass = 0; pass < n_pass; pass++) {
+    if (pass < my_pass) {
         float cur[T][4];
 #pragma unroll
         for (int t = 0; t < T; t++) {
@@ -239,9 +239,11 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
                 cur[t][s] = raw_value<Tin>(nxt[t][s]);
             }
         }
-        step_block(nby_, nbx_);
-        if (b + kWaves < nblk) {
-            fetch(nby_, nbx_, nxt);
+        {
+            const bool row_end = pass + 1 == my_pass;
+            if (!row_end || r + 1 < nby) {
+                fetch(row_end ? r + 1 : r, (row_end ? 0 : pass + 1) * kWaves + wave, nxt);
+            }
         }
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
@@ -393,15 +395,46 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(LaunchArgs a) {
         const floatx4 f1 = mm16<PREC>(make_opnd<PREC>(v.x, v.y, v.z, v.w), GXo, zero);
         const floatx4 o = mm16<PREC>(GYo, make_opnd<PREC>(f1[0], f1[1], f1[2], f1[3]), zero);
 
-        // ---- synthesis window + overlap-add into the tile accumulator ----
-        float* arow = acc + (bly * S + 4 * lg) * rw + blx * S + l16;
+        // ---- synthesis window, block output staged in LDS as [y][x] at offset wave*S ----
+        wave_lds_sync(); // V is read before the staged output overwrites it
+        float* srow = wbuf + wave * S + 4 * lg * 16 + l16;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            atomicAdd(arow + i * rw, SEP ? o[i] : o[i] * syn[i]);
+            srow[i * 16] = SEP ? o[i] : o[i] * syn[i];
         }
-        wave_lds_sync();
     }
-    __syncthreads();
+        // ---- overlap-add of pass `pass` of block row r, without atomics ----
+        // Every pixel (y, x) of the pass's 16-row band sums the staged outputs of the
+        // pass's blocks covering it; the thread that owns the pixel in this phase adds the
+        // sum to the accumulator.  With x local to the pass (origin: its first block), the
+        // value of the pass's block k for pixel x sits at stage[k*kWbuf + y*16 + x] (the
+        // staging offset k*S cancels the block origin), so a wave's lanes read consecutive
+        // banks.
+        __syncthreads();
+        {
+            const int Sg = opaque_uniform(S); // as fx in fetch: per-lane bounds stay in the loop
+            const int nb = min(kWaves, nbx - pass * kWaves);
+            const int cols = Sg * (nb - 1) + 16;
+#pragma unroll
+            for (int yy = 0; yy < 16; yy += kWaves) {
+                const int y = yy + wave;
+                const float* sy = stage + y * 16;
+                float* arow = acc + (r * Sg + y) * rw + pass * kWaves * Sg;
+                for (int x = lane; x < cols; x += 64) {
+                    // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)
+                    const int lo = x >= 15 ? (x - 15 + Sg - 1) / Sg : 0;
+                    const int hi = min(x / Sg, nb - 1);
+                    float sum = 0.0f;
+                    for (int k = lo; k <= hi; k++) {
+                        sum += sy[k * kWbuf + x];
+                    }
+                    arow[x] += sum;
+                }
+            }
+        }
+        __syncthreads(); // staged outputs consumed before the next pass reuses the buffers
+    }
+    }
 
     float* out = a.scratch + static_cast<long long>(item) * (a.region_h * rw);
     for (int i = tid; i < rows_used * rw; i += kThreads) {

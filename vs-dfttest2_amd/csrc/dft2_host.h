@@ -146,6 +146,16 @@ int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDe
                 ChunkPlan& out);
 
 int ensure_device(void** ptr, size_t* cap, size_t bytes);
+
+// The stream a call runs on: the caller's `stream`, HIP's null stream for
+// DFT2_DEFAULT_STREAM (a value HIP calls do not all accept), or `own` for NULL (then the
+// call also waits for completion).
+inline hipStream_t call_stream(void* stream, hipStream_t own) {
+    if (!stream) {
+        return own;
+    }
+    return stream == DFT2_DEFAULT_STREAM ? nullptr : static_cast<hipStream_t>(stream);
+}
 int select_device(const dft2_plan* p);
 
 } // namespace host

@@ -227,6 +227,21 @@ def rdft(data: Sequence[float], shape: Sequence[int]) -> np.ndarray:
     return out[:n_out]
 
 
+# DFT2_DEFAULT_STREAM: the null (default) stream as a stream argument.  torch's default
+# stream has handle 0, which the C ABI would take as "no stream" (the plan's own stream,
+# and a blocking call).
+DFT2_DEFAULT_STREAM = 1
+
+
+def _stream_arg(stream: Optional[int]) -> ctypes.c_void_p:
+    """None: the plan's (or batch's) own stream, and the call waits for completion.
+    Otherwise a hipStream_t handle such as a torch stream's cuda_stream, where 0 (torch's
+    default stream) is passed as DFT2_DEFAULT_STREAM: work is queued, not waited for."""
+    if stream is None:
+        return ctypes.c_void_p(0)
+    return ctypes.c_void_p(int(stream) or DFT2_DEFAULT_STREAM)
+
+
 class Plan:
     """One set of DFTTest plugin parameters on one GPU (dft2_plan_create)."""
 
@@ -285,26 +300,26 @@ class Plan:
 
     def process_batch(self, jobs: List[Dft2Job], stream: Optional[int] = None):
         arr = (Dft2Job * len(jobs))(*jobs)
-        _check(load().dft2_process_batch(self._h, arr, len(jobs), ctypes.c_void_p(stream or 0)))
+        _check(load().dft2_process_batch(self._h, arr, len(jobs), _stream_arg(stream)))
 
     def process_batch_sigma(self, jobs: List[Dft2Job], sigma_ptr: int, sigma_scale: float,
                             stream: Optional[int] = None):
         arr = (Dft2Job * len(jobs))(*jobs)
         _check(load().dft2_process_batch_sigma(self._h, arr, len(jobs), ctypes.c_void_p(sigma_ptr),
-                                               ctypes.c_float(sigma_scale), ctypes.c_void_p(stream or 0)))
+                                               ctypes.c_float(sigma_scale), _stream_arg(stream)))
 
     def backward_batch(self, jobs: List[Dft2GradJob], sigma_ptr: int, sigma_scale: float, sigma_grad_ptr: int,
                        stream: Optional[int] = None):
         arr = (Dft2GradJob * len(jobs))(*jobs)
         _check(load().dft2_backward_batch(self._h, arr, len(jobs), ctypes.c_void_p(sigma_ptr),
                                           ctypes.c_float(sigma_scale), ctypes.c_void_p(sigma_grad_ptr),
-                                          ctypes.c_void_p(stream or 0)))
+                                          _stream_arg(stream)))
 
     def process_frame(self, src_ptrs: Sequence[int], src_stride: int, dst: int, dst_stride: int, width: int,
                       height: int, memory: int, stream: Optional[int] = None):
         ptrs = (ctypes.c_void_p * len(src_ptrs))(*src_ptrs)
         _check(load().dft2_process_frame(self._h, ptrs, src_stride, ctypes.c_void_p(dst), dst_stride, width, height,
-                                         memory, ctypes.c_void_p(stream or 0)))
+                                         memory, _stream_arg(stream)))
 
 
 class Batch:
@@ -327,7 +342,7 @@ class Batch:
         self._h = handle
 
     def launch(self, stream: Optional[int] = None):
-        _check(load().dft2_batch_launch(self._h, ctypes.c_void_p(stream or 0)))
+        _check(load().dft2_batch_launch(self._h, _stream_arg(stream)))
 
     def info(self) -> dict:
         out = Dft2BatchInfo()
