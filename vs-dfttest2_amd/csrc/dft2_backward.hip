@@ -495,12 +495,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
                     const int g = (r * Sg + y) * rw + pass * kWaves * Sg + x;
 #pragma unroll
                     for (int t = 0; t < T; t++) {
-                        const float* sy = stage + t * 256 + y * 16 + x;
-                        float sum = 0.0f;
-                        for (int k = lo; k <= hi; k++) {
-                            sum += sy[k * kWbuf];
-                        }
-                        gacc[t * region + g] += sum;
+                        gacc[t * region + g] += gather_sum(stage + t * 256 + y * 16 + x, kWbuf, lo, hi);
                     }
                 }
             }

@@ -151,12 +151,15 @@ int time_runs(dft2_batch* b, int persistent, int reps, float* ms) {
 
 int autotune(dft2_batch* b, int sample_frames) {
     dft2_plan* p = b->plan;
-    static const int kSides[] = {16, 12, 8, 6, 4};
+    // Widths in whole passes of kWaves blocks (every wave busy), heights from tall to
+    // short; the plan's own choice is among them.
+    static const int kWidths[] = {8, 16};
+    static const int kHeights[] = {24, 16, 12, 8, 4};
     float best = -1.0f;
     TileShape best_ts = p->tile;
     int best_persistent = 0;
-    for (int bh : kSides) {
-        for (int bw : kSides) {
+    for (int bh : kHeights) {
+        for (int bw : kWidths) {
             if (!tile_valid(p, bw, bh)) {
                 continue;
             }

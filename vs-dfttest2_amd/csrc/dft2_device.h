@@ -105,6 +105,24 @@ __device__ __forceinline__ int opaque_uniform(int v) {
     return v;
 }
 
+// Overlap-add gather: the sum of the staged values of blocks lo..hi covering one pixel,
+// block k's value at p[k * stride].  Up to four (all of them when the block step is at
+// least 4) are read together, the rest one by one.  Reads for k < lo stay inside the
+// staging area and are discarded by the select.
+__device__ __forceinline__ float gather_sum(const float* p, int stride, int lo, int hi) {
+    float sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int k = hi - j;
+        const float v = p[max(k, 0) * stride];
+        sum += k >= lo ? v : 0.0f;
+    }
+    for (int k = hi - 4; k >= lo; k--) {
+        sum += p[k * stride];
+    }
+    return sum;
+}
+
 __device__ __forceinline__ int mirror(int v, int n) {
     v = v < 0 ? -v : v;
     return v >= n ? 2 * (n - 1) - v : v;
