@@ -27,10 +27,10 @@ METRIC = "Mpx/s at 1080p & 4K, plain FP32 vs split-BF16 MFMA at ≤1 LSB; fwd+bw
 WIDTH, HEIGHT, FRAMES = 1920, 1080, 240
 PEAK_FP32_MFMA_TFLOPS = 157.3
 # HBM bytes per fused launch on the headline workload (one launch per step): rocprofv3
-# --pmc FETCH_SIZE 3,493,600 KiB x 2 + WRITE_SIZE 4,603,720 KiB x 1, the factors being what
+# --pmc FETCH_SIZE 4,553,170 KiB x 2 + WRITE_SIZE 4,797,210 KiB x 1, the factors being what
 # the same counters read for a plain copy kernel of known size
 # (profiles/fused_kernel_1080p_yuv420_u8_fp32.md).  Valid for the FP32 default workload.
-FUSED_TRAFFIC_BYTES = (3_493_600 * 2 + 4_603_720) * 1024
+FUSED_TRAFFIC_BYTES = (4_553_170 * 2 + 4_797_210) * 1024
 
 
 def flop_per_block(t_len: int) -> int:

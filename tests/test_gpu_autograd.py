@@ -41,10 +41,12 @@ def test_diff_forward_matches_oracle(tbsize):
     assert float(np.max(np.abs(y.cpu().numpy() - ref))) <= 1e-5
 
 
-@pytest.mark.parametrize("tbsize,shape", [(3, (2, 5, 40, 44)), (1, (1, 3, 36, 36)), (5, (1, 6, 34, 40))])
-def test_gradients_match_fp64_autograd(tbsize, shape):
-    announce("diff backward", tbsize, shape)
-    p = diff_params(tbsize=tbsize)
+@pytest.mark.parametrize("tbsize,shape,sosize", [(3, (2, 5, 40, 44), 12), (1, (1, 3, 36, 36), 12), (5, (1, 6, 34, 40), 12),
+                                                 (3, (1, 4, 38, 42), 14), (3, (1, 4, 44, 36), 8)])
+def test_gradients_match_fp64_autograd(tbsize, shape, sosize):
+    # block steps 4 (default), 2 (eight blocks cover a pixel per axis) and 8
+    announce("diff backward", tbsize, shape, sosize)
+    p = diff_params(tbsize=tbsize, sosize=sosize)
     b, n, h, w = shape
     clean = torch.stack([synth.clean_plane(n, h, w, 30 + i) for i in range(b)])
     noisy = clean + torch.stack([synth.noise((n, h, w), 0.05, 40 + i) for i in range(b)])

@@ -73,6 +73,8 @@ CASES = [
     ("u8", (5, 47, 53), dict(sosize=14)),
     ("u8", (5, 47, 53), dict(sosize=0)),
     ("f32", (4, 40, 40), dict(sosize=4, swin=9)),
+    ("u8", (4, 38, 46), dict(sosize=15)),  # block step 1: 16 blocks per axis cover a pixel
+    ("f32", (4, 45, 50), dict(sosize=7, tbsize=5)),  # block step 9
     ("u8", (4, 44, 44), dict(ssystem=1, slocation=[0.0, 2.0, 1.0, 30.0])),
 ]
 

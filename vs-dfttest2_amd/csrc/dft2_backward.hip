@@ -483,6 +483,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
         __syncthreads();
         {
             const int Sg = opaque_uniform(S); // per-lane bounds stay in the loop
+            const int Sm = (65536 + Sg - 1) / Sg; // for div_small
             const int nb = min(kWaves, nbx - pass * kWaves);
             const int cols = Sg * (nb - 1) + 16;
 #pragma unroll
@@ -490,12 +491,12 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
                 const int y = yy + wave;
                 for (int x = lane; x < cols; x += 64) {
                     // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)
-                    const int lo = x >= 15 ? (x - 15 + Sg - 1) / Sg : 0;
-                    const int hi = min(x / Sg, nb - 1);
+                    const int lo = x >= 15 ? div_small(x - 15 + Sg - 1, Sm) : 0;
+                    const int hi = min(div_small(x, Sm), nb - 1);
                     const int g = (r * Sg + y) * rw + pass * kWaves * Sg + x;
 #pragma unroll
                     for (int t = 0; t < T; t++) {
-                        gacc[t * region + g] += gather_sum(stage + t * 256 + y * 16 + x, kWbuf, lo, hi);
+                        gacc[t * region + g] += gather_sum<kWbuf>(stage + t * 256 + y * 16 + x, lo, hi);
                     }
                 }
             }

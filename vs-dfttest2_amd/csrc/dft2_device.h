@@ -105,20 +105,30 @@ __device__ __forceinline__ int opaque_uniform(int v) {
     return v;
 }
 
+// x / S for 0 <= x < 256 and 1 <= S <= 16, given m = ceil(65536 / S): exact, since
+// x * m / 65536 exceeds x / S by less than 256 / 65536 < 1 / S.
+__device__ __forceinline__ int div_small(int x, int m) {
+    return (x * m) >> 16;
+}
+
 // Overlap-add gather: the sum of the staged values of blocks lo..hi covering one pixel,
-// block k's value at p[k * stride].  Up to four (all of them when the block step is at
-// least 4) are read together, the rest one by one.  Reads for k < lo stay inside the
-// staging area and are discarded by the select.
-__device__ __forceinline__ float gather_sum(const float* p, int stride, int lo, int hi) {
+// block k's value at p[k * kStride].  The four blocks k0..k0+3, k0 = max(hi - 3, 0), are
+// read together at immediate offsets (for block steps >= 4 they include every covering
+// block) and masked to lo..hi; block steps 1 and 2 add the rest one by one.  Every read
+// stays below block max(hi, 3) <= 7, inside the staging area.
+template <int kStride>
+__device__ __forceinline__ float gather_sum(const float* p, int lo, int hi) {
+    const int k0 = max(hi - 3, 0);
+    const float* q = p + k0 * kStride;
     float sum = 0.0f;
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-        const int k = hi - j;
-        const float v = p[max(k, 0) * stride];
-        sum += k >= lo ? v : 0.0f;
+        const float v = q[j * kStride];
+        // lo <= k0 + j <= hi as one compare, so that the read is not made conditional
+        sum += static_cast<unsigned>(k0 + j - lo) <= static_cast<unsigned>(hi - lo) ? v : 0.0f;
     }
-    for (int k = hi - 4; k >= lo; k--) {
-        sum += p[k * stride];
+    for (int k = k0 - 1; k >= lo; k--) {
+        sum += p[k * kStride];
     }
     return sum;
 }

@@ -430,6 +430,7 @@ void fused_kernel(LaunchArgs a) {
         __syncthreads();
         {
             const int Sg = opaque_uniform(S); // as fx in fetch: per-lane bounds stay in the loop
+            const int Sm = (65536 + Sg - 1) / Sg; // for div_small
             const int nb = min(kWaves, nbx - pass * kWaves);
             const int cols = Sg * (nb - 1) + 16;
 #pragma unroll
@@ -439,9 +440,9 @@ void fused_kernel(LaunchArgs a) {
                 float* arow = acc + (r * Sg + y) * rw + pass * kWaves * Sg;
                 for (int x = lane; x < cols; x += 64) {
                     // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)
-                    const int lo = x >= 15 ? (x - 15 + Sg - 1) / Sg : 0;
-                    const int hi = min(x / Sg, nb - 1);
-                    arow[x] += gather_sum(sy + x, kWbuf, lo, hi);
+                    const int lo = x >= 15 ? div_small(x - 15 + Sg - 1, Sm) : 0;
+                    const int hi = min(div_small(x, Sm), nb - 1);
+                    arow[x] += gather_sum<kWbuf>(sy + x, lo, hi);
                 }
             }
         }
