@@ -52,8 +52,11 @@ void fused_kernel(LaunchArgs a) {
     // the barriers cost more than the atomics; 4K, tbsize 5: 232 against 209 ms).
     constexpr bool kStagedOla = R <= 1;
     // Per-wave LDS: the T*256 spectrum staging, reused for V (16 x 20) and for the block
-    // output staged at offset wave*S (<= 7*16 floats) for the overlap-add.
-    constexpr int kWbuf = (T * 256 > 384) ? T * 256 : 384;
+    // output, staged with row pitch kOut at offset wave*S (<= 7*16 floats) for the
+    // overlap-add: 16 rows * 20 + 112 <= 448 floats.  Pitch 20 puts the four lane groups'
+    // rows 4*(lane/16)+i in different banks.
+    constexpr int kOut = 20;
+    constexpr int kWbuf = (T * 256 > 448) ? T * 256 : 448;
     extern __shared__ float smem[];
 
     const int tid = threadIdx.x;
@@ -402,10 +405,10 @@ void fused_kernel(LaunchArgs a) {
         if constexpr (kStagedOla) {
             // ---- synthesis window, block output staged in LDS as [y][x] at offset wave*S ----
             wave_lds_sync(); // V is read before the staged output overwrites it
-            float* srow = wbuf + wave * S + 4 * lg * 16 + l16;
+            float* srow = wbuf + wave * S + 4 * lg * kOut + l16;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                srow[i * 16] = SEP ? o[i] : o[i] * syn[i];
+                srow[i * kOut] = SEP ? o[i] : o[i] * syn[i];
             }
         } else {
             // ---- synthesis window + overlap-add by LDS atomics ----
@@ -424,7 +427,7 @@ void fused_kernel(LaunchArgs a) {
         // Every pixel (y, x) of the pass's 16-row band sums the staged outputs of the
         // pass's blocks covering it; the thread that owns the pixel in this phase adds the
         // sum to the accumulator.  With x local to the pass (origin: its first block), the
-        // value of the pass's block k for pixel x sits at stage[k*kWbuf + y*16 + x] (the
+        // value of the pass's block k for pixel x sits at stage[k*kWbuf + y*kOut + x] (the
         // staging offset k*S cancels the block origin), so a wave's lanes read consecutive
         // banks.
         __syncthreads();
@@ -436,7 +439,7 @@ void fused_kernel(LaunchArgs a) {
 #pragma unroll
             for (int yy = 0; yy < 16; yy += kWaves) {
                 const int y = yy + wave;
-                const float* sy = stage + y * 16;
+                const float* sy = stage + y * kOut;
                 float* arow = acc + (r * Sg + y) * rw + pass * kWaves * Sg;
                 for (int x = lane; x < cols; x += 64) {
                     // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)

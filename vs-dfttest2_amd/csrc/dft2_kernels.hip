@@ -116,7 +116,7 @@ hipError_t allow_lds(const void* kern, size_t lds) {
 
 size_t fused_lds_bytes(int radius, int region_w, int region_h) {
     const int t_len = 2 * radius + 1;
-    const int wbuf = (t_len * 256 > 384) ? t_len * 256 : 384; // as kWbuf in the fused kernel
+    const int wbuf = (t_len * 256 > 448) ? t_len * 256 : 448; // as kWbuf in the fused kernel
     return sizeof(float) * (static_cast<size_t>(region_w) * region_h + (radius + 1) * 4 * 64 * 4 + kWaves * wbuf);
 }
 
