@@ -194,6 +194,83 @@ __device__ __forceinline__ float raw_value(raw_t<Tin> v) {
     }
 }
 
+// Four consecutive samples (columns 4*(lane/16)..+3) of one lane and slice as raw bits:
+// one dword of 4 bytes, two dwords of 2 halves, or four float dwords.  They stay packed
+// until the samples are used, so a prefetch keeps its loads in flight and holds T
+// registers per lane for 8-bit samples instead of 4*T.
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename Tin>
+struct raw4 {
+    unsigned v[sizeof(Tin)];
+};
+
+// Byte alignment of a raw4 load as one access (dword, dwordx2, dwordx4): its own size.
+template <typename Tin>
+constexpr unsigned kRaw4Align = 4u * sizeof(Tin);
+
+// The four samples at byte offset `off`, which is a multiple of kRaw4Align<Tin>.
+template <typename Tin>
+__device__ __forceinline__ raw4<Tin> load4_aligned(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    raw4<Tin> x;
+    if constexpr (sizeof(Tin) == 1) {
+        x.v[0] = __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+    } else if constexpr (sizeof(Tin) == 2) {
+        const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0);
+        x.v[0] = w.x;
+        x.v[1] = w.y;
+    } else {
+        const u32x4 w = __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+        x.v[0] = w.x;
+        x.v[1] = w.y;
+        x.v[2] = w.z;
+        x.v[3] = w.w;
+    }
+    return x;
+}
+
+// The four samples at byte offsets off[0..3], one load each.  Integer samples are packed
+// as they arrive (their loads complete here); float samples stay in flight.
+template <typename Tin>
+__device__ __forceinline__ raw4<Tin> load4_each(__amdgpu_buffer_rsrc_t r, const unsigned (&off)[4]) {
+    raw4<Tin> x;
+    if constexpr (sizeof(Tin) == 1) {
+        unsigned b[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            b[s] = __builtin_amdgcn_raw_buffer_load_b8(r, off[s], 0, 0);
+        }
+        x.v[0] = b[0] | (b[1] << 8) | (b[2] << 16) | (b[3] << 24);
+    } else if constexpr (sizeof(Tin) == 2) {
+        unsigned h[4];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            h[s] = __builtin_amdgcn_raw_buffer_load_b16(r, off[s], 0, 0);
+        }
+        x.v[0] = h[0] | (h[1] << 16);
+        x.v[1] = h[2] | (h[3] << 16);
+    } else {
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            x.v[s] = __builtin_amdgcn_raw_buffer_load_b32(r, off[s], 0, 0);
+        }
+    }
+    return x;
+}
+
+// Sample s (0..3) of a raw4 as float: one conversion instruction for integer samples.
+template <typename Tin>
+__device__ __forceinline__ float raw4_value(const raw4<Tin>& x, int s) {
+    if constexpr (sizeof(Tin) == 1) {
+        return static_cast<float>((x.v[0] >> (8 * s)) & 0xffu);
+    } else if constexpr (sizeof(Tin) == 2) {
+        return static_cast<float>((x.v[s >> 1] >> (16 * (s & 1))) & 0xffffu);
+    } else {
+        return __builtin_bit_cast(float, x.v[s]);
+    }
+}
+
 // Source planes live in device global memory; loading through an address-space-1
 // pointer gives global (not flat) loads.
 template <typename Tin>

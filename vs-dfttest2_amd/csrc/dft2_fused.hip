@@ -173,7 +173,11 @@ void fused_kernel(LaunchArgs a) {
     }
 
     // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
+    // wide_ok: every plane base and the row pitch are multiples of kRaw4Align, so blocks
+    // whose first sample is too take one wide load per slice.
     __amdgpu_buffer_rsrc_t frame[T];
+    constexpr unsigned kAl = kRaw4Align<Tin>;
+    bool wide_ok = (static_cast<unsigned>(rs) * sizeof(Tin)) % kAl == 0;
     {
         const int n_frames = sgpr(job.n_frames);
         const bool use_fptr = sgpr(job.use_fptr) != 0;
@@ -186,24 +190,30 @@ void fused_kernel(LaunchArgs a) {
             const Tin* base = use_fptr ? sgpr_ptr(reinterpret_cast<const Tin*>(job.fptr[fi]))
                                        : src + static_cast<long long>(fi) * fp;
             frame[t] = frame_rsrc(base, frame_bytes);
+            wide_ok = wide_ok && (reinterpret_cast<unsigned long long>(base) % kAl) == 0;
         }
     }
     // Samples of block (fy, fx) of the tile for this lane: row y = lane%16, columns
     // 4*(lane/16)..+3 of every slice, as raw bits.  Blocks that lie inside the plane
-    // (most of them) take one offset per lane; the others apply the mirror padding per
-    // sample.
+    // (most of them) take one offset per lane, and one wide load per slice when aligned;
+    // the others apply the mirror padding per sample.
     constexpr unsigned kSz = sizeof(Tin);
-    auto fetch = [&](int fy, int fx, raw_t<Tin> (&dst)[T][4]) {
+    auto fetch = [&](int fy, int fx, raw4<Tin> (&dst)[T]) {
         fx = opaque_uniform(fx); // a wave's column is fixed: keep its offsets out of the loop's registers
         const int y0 = (by0 + fy) * S - oy;
         const int x0 = (bx0 + fx) * S - ox;
         if (y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width) {
             const unsigned off = (static_cast<unsigned>(y0 * rs + x0) + lane_off) * kSz;
+            if (wide_ok && (static_cast<unsigned>(y0 * rs + x0) * kSz) % kAl == 0) {
 #pragma unroll
-            for (int t = 0; t < T; t++) {
+                for (int t = 0; t < T; t++) {
+                    dst[t] = load4_aligned<Tin>(frame[t], off);
+                }
+            } else {
+                const unsigned offs[4] = {off, off + kSz, off + 2 * kSz, off + 3 * kSz};
 #pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    dst[t][s] = load_raw<Tin>(frame[t], off + s * kSz);
+                for (int t = 0; t < T; t++) {
+                    dst[t] = load4_each<Tin>(frame[t], offs);
                 }
             }
         } else {
@@ -215,10 +225,7 @@ void fused_kernel(LaunchArgs a) {
             }
 #pragma unroll
             for (int t = 0; t < T; t++) {
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    dst[t][s] = load_raw<Tin>(frame[t], offs[s]);
-                }
+                dst[t] = load4_each<Tin>(frame[t], offs);
             }
         }
     };
@@ -228,7 +235,7 @@ void fused_kernel(LaunchArgs a) {
     // so its load latency overlaps the current block's transforms.
     const int n_pass = (nbx + kWaves - 1) / kWaves;
     const int my_pass = nbx > wave ? (nbx - wave + kWaves - 1) / kWaves : 0; // passes with a block of this wave
-    raw_t<Tin> nxt[T][4];
+    raw4<Tin> nxt[T];
     if (my_pass > 0) {
         fetch(0, wave, nxt);
     }
@@ -243,7 +250,7 @@ void fused_kernel(LaunchArgs a) {
         for (int t = 0; t < T; t++) {
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                cur[t][s] = raw_value<Tin>(nxt[t][s]);
+                cur[t][s] = raw4_value<Tin>(nxt[t], s);
             }
         }
         {
