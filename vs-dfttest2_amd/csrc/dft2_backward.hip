@@ -158,6 +158,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
 
     __amdgpu_buffer_rsrc_t frame[T];
     float* gin[T];
+    // wide_ok: plane bases and row pitches allow one dwordx4 per lane and slice for blocks
+    // whose first sample is 16-byte aligned too.
+    bool wide_ok = (rs % 4) == 0 && (go_rs % 4) == 0;
     {
         const int n_frames = sgpr(job.n_frames);
         const float* src = sgpr_ptr(reinterpret_cast<const float*>(job.src));
@@ -168,13 +171,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const int fi = min(max(n - R + t, 0), n_frames - 1);
-            frame[t] = frame_rsrc(src + static_cast<long long>(fi) * fp, frame_bytes);
+            const float* base = src + static_cast<long long>(fi) * fp;
+            frame[t] = frame_rsrc(base, frame_bytes);
             gin[t] = gi + static_cast<long long>(fi) * gfp;
+            wide_ok = wide_ok && (reinterpret_cast<unsigned long long>(base) % 16) == 0;
         }
     }
-    const __amdgpu_buffer_rsrc_t gout =
-        frame_rsrc(sgpr_ptr(reinterpret_cast<const float*>(job.grad_out)) + static_cast<long long>(n) * sgpr64(job.go_fp),
-                   static_cast<unsigned>(go_rs * height) * 4u);
+    const float* go_base =
+        sgpr_ptr(reinterpret_cast<const float*>(job.grad_out)) + static_cast<long long>(n) * sgpr64(job.go_fp);
+    const __amdgpu_buffer_rsrc_t gout = frame_rsrc(go_base, static_cast<unsigned>(go_rs * height) * 4u);
+    wide_ok = wide_ok && (reinterpret_cast<unsigned long long>(go_base) % 16) == 0;
     const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg) * 4u;
     const unsigned lane_off_go = static_cast<unsigned>(l16 * go_rs + 4 * lg) * 4u;
 
@@ -194,26 +200,31 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
     }
 
     // Samples of the T slices and of the output gradient for block (fy, fx) of the tile,
-    // as raw bits, fetched one block ahead (once the current block's samples are
-    // consumed) so that their latency overlaps the current block's work.  Output-gradient samples outside the plane count as 0: their validity
-    // is kept as a 4-bit mask (bit s for column 4*(lane/16) + s) and applied at use.
-    auto fetch = [&](int fy, int fx, unsigned (&dst)[T][4], unsigned (&dgo)[4], unsigned& okm) {
+    // as raw bits, fetched one block ahead (once the current block's samples are consumed)
+    // so that their latency overlaps the current block's work; aligned interior blocks take
+    // one dwordx4 per slice.  Output-gradient samples outside the plane count as 0: their
+    // validity is kept as a 4-bit mask (bit s for column 4*(lane/16) + s) and applied at use.
+    auto fetch = [&](int fy, int fx, raw4<float> (&dst)[T], raw4<float>& dgo, unsigned& okm) {
         fx = opaque_uniform(fx); // a wave's column is fixed: keep its offsets out of the loop's registers
         const int y0 = (by0 + fy) * S - oy;
         const int x0 = (bx0 + fx) * S - ox;
         if (y0 >= 0 && y0 + 15 < height && x0 >= 0 && x0 + 15 < width) {
             const unsigned off = static_cast<unsigned>(y0 * rs + x0) * 4u + lane_off;
-#pragma unroll
-            for (int t = 0; t < T; t++) {
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    dst[t][s] = load_raw<float>(frame[t], off + 4u * s);
-                }
-            }
             const unsigned og = static_cast<unsigned>(y0 * go_rs + x0) * 4u + lane_off_go;
+            if (wide_ok && ((y0 * rs + x0) % 4) == 0 && ((y0 * go_rs + x0) % 4) == 0) {
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                dgo[s] = load_raw<float>(gout, og + 4u * s);
+                for (int t = 0; t < T; t++) {
+                    dst[t] = load4_aligned<float>(frame[t], off);
+                }
+                dgo = load4_aligned<float>(gout, og);
+            } else {
+                const unsigned offs[4] = {off, off + 4u, off + 8u, off + 12u};
+                const unsigned ogs[4] = {og, og + 4u, og + 8u, og + 12u};
+#pragma unroll
+                for (int t = 0; t < T; t++) {
+                    dst[t] = load4_each<float>(frame[t], offs);
+                }
+                dgo = load4_each<float>(gout, ogs);
             }
             okm = 0xfu;
         } else {
@@ -225,28 +236,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             }
 #pragma unroll
             for (int t = 0; t < T; t++) {
-#pragma unroll
-                for (int s = 0; s < 4; s++) {
-                    dst[t][s] = load_raw<float>(frame[t], offs[s]);
-                }
+                dst[t] = load4_each<float>(frame[t], offs);
             }
             const int gy = y0 + l16;
             const bool row_ok = gy >= 0 && gy < height;
             okm = 0u;
+            unsigned ogs[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 const int gx = x0 + 4 * lg + s;
                 const bool ok = row_ok && gx >= 0 && gx < width;
-                dgo[s] = load_raw<float>(gout, ok ? static_cast<unsigned>(gy * go_rs + gx) * 4u : 0u);
+                ogs[s] = ok ? static_cast<unsigned>(gy * go_rs + gx) * 4u : 0u;
                 okm |= ok ? (1u << s) : 0u;
             }
+            dgo = load4_each<float>(gout, ogs);
         }
     };
     // Each block row r of the tile is done in passes of kWaves blocks: in pass p wave w
     // takes block (r, p*kWaves + w).  The wave's next block is fetched one block ahead.
     const int n_pass = (nbx + kWaves - 1) / kWaves;
     const int my_pass = nbx > wave ? (nbx - wave + kWaves - 1) / kWaves : 0; // passes with a block of this wave
-    unsigned nxt[T][4], nxt_go[4], nxt_ok = 0u;
+    raw4<float> nxt[T], nxt_go;
+    unsigned nxt_ok = 0u;
     if (my_pass > 0) {
         fetch(0, wave, nxt, nxt_go, nxt_ok);
     }
@@ -264,7 +275,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             float wv[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                wv[s] = win[t][s] * raw_value<float>(nxt[t][s]);
+                wv[s] = win[t][s] * raw4_value<float>(nxt[t], s);
             }
             const Opnd in = make_opnd<kBP>(wv[0], wv[1], wv[2], wv[3]);
             const floatx4 d1 = mm16<kBP>(in, WXo, zero);
@@ -311,7 +322,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             float dov[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                dov[s] = ((nxt_ok >> s) & 1u) ? raw_value<float>(nxt_go[s]) * synA[s] : 0.0f;
+                dov[s] = ((nxt_ok >> s) & 1u) ? raw4_value<float>(nxt_go, s) * synA[s] : 0.0f;
             }
             // The samples of this block are consumed: fetch the wave's next block into the
             // same registers; the loads have the rest of this block's work to complete.
