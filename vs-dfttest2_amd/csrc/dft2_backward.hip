@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
     // fraction of the FP32 MFMA cost.
     constexpr int kBP = 1;
     constexpr int kWbuf = T * 256 + 256;
-    extern __shared__ float smem[];
+    extern __shared__ __attribute__((aligned(16))) float smem[]; // float4 accesses
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;

@@ -57,7 +57,7 @@ void fused_kernel(LaunchArgs a) {
     // rows 4*(lane/16)+i in different banks.
     constexpr int kOut = 20;
     constexpr int kWbuf = (T * 256 > 448) ? T * 256 : 448;
-    extern __shared__ float smem[];
+    extern __shared__ __attribute__((aligned(16))) float smem[]; // float4 accesses
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -168,8 +168,10 @@ void fused_kernel(LaunchArgs a) {
     const int rs = sgpr(static_cast<int>(job.src_rs));
     const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg);
 
-    for (int i = tid; i < rows_used * rw; i += kThreads) {
-        acc[i] = 0.0f;
+    // The region (rows_used x rw, rw a multiple of 4) is zeroed and read out as float4.
+    float4* acc4 = reinterpret_cast<float4*>(acc);
+    for (int i = tid; i < rows_used * rw / 4; i += kThreads) {
+        acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 
     // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
@@ -463,9 +465,9 @@ void fused_kernel(LaunchArgs a) {
         __syncthreads(); // every block's atomics are done before the region is read out
     }
 
-    float* out = a.scratch + static_cast<long long>(item) * (a.region_h * rw);
-    for (int i = tid; i < rows_used * rw; i += kThreads) {
-        out[i] = acc[i];
+    float4* out = reinterpret_cast<float4*>(a.scratch + static_cast<long long>(item) * (a.region_h * rw));
+    for (int i = tid; i < rows_used * rw / 4; i += kThreads) {
+        out[i] = acc4[i];
     }
     if constexpr (!PERSIST) {
         break;

@@ -16,6 +16,27 @@ __device__ __forceinline__ void store_sample(Tout* p, float v, const LaunchArgs&
     }
 }
 
+// Four consecutive converted samples to p, aligned to 4 * sizeof(Tout) bytes: one store.
+template <typename Tout>
+__device__ __forceinline__ void store_samples4(Tout* p, const float (&v)[4], const LaunchArgs& a) {
+    if constexpr (sizeof(Tout) == 4) {
+        *reinterpret_cast<float4*>(p) =
+            make_float4(v[0] / a.out_scale, v[1] / a.out_scale, v[2] / a.out_scale, v[3] / a.out_scale);
+    } else {
+        unsigned q[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const float x = fmaxf(0.0f, fminf(v[e] / a.out_scale + 0.5f, static_cast<float>(a.peak)));
+            q[e] = static_cast<unsigned>(static_cast<int>(x));
+        }
+        if constexpr (sizeof(Tout) == 1) {
+            *reinterpret_cast<unsigned*>(p) = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+        } else {
+            *reinterpret_cast<uint2*>(p) = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+        }
+    }
+}
+
 // One workgroup per work item (frame, tile): the output pixels the tile owns (its S*tb x
 // S*tb corner of the padded plane; the last tile in a row or column owns its whole
 // region) take the tile's own region plus the 16 - S overlapping columns / rows of the
@@ -61,8 +82,75 @@ __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
     Tout* dst = reinterpret_cast<Tout*>(sgpr_ptr(job.dst)) +
                 static_cast<long long>(sgpr(job.f_begin) + f_local) * sgpr64(job.dst_fp);
     const long long drs = sgpr64(job.dst_rs);
-    const int w = x1 - x0;
     const int h = y1 - y0;
+    // Columns in groups of four starting at multiples of 4 (padded coordinates): one float4
+    // read per region and one store per group, when the tile origin, the padding offset, the
+    // tile width and the output rows are aligned for it; sums in the same order as below.
+    if (((x_lo | ox | tw) & 3) == 0 && (drs & 3) == 0 &&
+        (reinterpret_cast<unsigned long long>(dst) % (4 * sizeof(Tout))) == 0) {
+        const int g0 = x0 >> 2;
+        const int ng = ((x1 + 3) >> 2) - g0;
+        const int dr4 = 256 / ng;
+        const int dc4 = 256 - dr4 * ng;
+        int r = threadIdx.x / ng;
+        int g = threadIdx.x - r * ng;
+        for (; r < h;) {
+            const int x = (g0 + g) * 4;
+            const int ly = y0 + r - y_lo;
+            const int lx = x - x_lo; // a multiple of 4, as are rw and tw: float4 reads
+            const float4 o4 = *reinterpret_cast<const float4*>(own + ly * rw + lx);
+            float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            const float ow[4] = {o4.x, o4.y, o4.z, o4.w};
+            const bool lft = has_left && lx < ov;
+            if (has_top && ly < ov) {
+                const float4 t4 = *reinterpret_cast<const float4*>(top + (ly + th) * rw + lx);
+                const float tp[4] = {t4.x, t4.y, t4.z, t4.w};
+                if (lft) {
+                    const float4 c4 = *reinterpret_cast<const float4*>(corner + (ly + th) * rw + lx + tw);
+                    const float cn[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        v[e] += lx + e < ov ? cn[e] : 0.0f;
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    v[e] += tp[e];
+                }
+            }
+            if (lft) {
+                const float4 l4 = *reinterpret_cast<const float4*>(left + ly * rw + lx + tw);
+                const float lf[4] = {l4.x, l4.y, l4.z, l4.w};
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    v[e] += lx + e < ov ? lf[e] : 0.0f;
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                v[e] += ow[e];
+            }
+            Tout* drow = dst + (y0 + r - oy) * drs;
+            if (x >= x0 && x + 4 <= x1) {
+                store_samples4(drow + (x - ox), v, a);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (x + e >= x0 && x + e < x1) {
+                        store_sample(drow + (x + e - ox), v[e], a);
+                    }
+                }
+            }
+            g += dc4;
+            r += dr4;
+            if (g >= ng) {
+                g -= ng;
+                ++r;
+            }
+        }
+        return;
+    }
+    const int w = x1 - x0;
     // pixel i of the w x h area is (r, c) = (i / w, i % w), stepped without a division
     const int dr = 256 / w;
     const int dc = 256 - dr * w;
