@@ -65,10 +65,11 @@ __device__ __forceinline__ Opnd make_opnd(float a0, float a1, float a2, float a3
 // PREC 0: four FP32 MFMA 16x16x4.  PREC 1: split BF16 with FP32 accumulation,
 // A*B ~ Ahi*Bhi + Alo*Bhi + Ahi*Blo, three BF16 MFMA 16x16x16.  The K slot pairing
 // (lane group, s) is the same in every form, so operands need no reshuffle.
-// The K-packed alternative (one 16x16x32 for [Ahi | Alo] x [Bhi | Bhi] plus one 16x16x16)
-// returned corrupted products on gfx950 when the compiler scheduled a VALU write to a
-// source register of the 16x16x32 right after it; with wait states inserted it was
-// correct, and three 16x16x16 ran at the same speed, so this form is used.
+// The K-packed alternative, one 16x16x32 on [Ahi | Alo] x [Blo | Bhi] plus one 16x16x16
+// on the hi halves, was measured on gfx950 with this toolchain: chained through one
+// accumulator (either order) it returned wrong products; with separate accumulators and
+// a VALU add it was correct but no faster than three 16x16x16 (1080p u8, 4K 10-bit and
+// the backward pass), so this form is used.
 template <int PREC>
 __device__ __forceinline__ floatx4 mm16(const Opnd& A, const Opnd& B, floatx4 acc) {
     if constexpr (PREC == 0) {
@@ -85,6 +86,21 @@ __device__ __forceinline__ floatx4 mm16(const Opnd& A, const Opnd& B, floatx4 ac
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
     }
+    return acc;
+}
+
+// Split-BF16 contraction whose A operand is exact in BF16 (integers of at most 8
+// significant bits): A*B = A*Bhi + A*Blo, two BF16 MFMA.  Only A.hi is set.
+__device__ __forceinline__ Opnd make_opnd_exact(float a0, float a1, float a2, float a3) {
+    Opnd o;
+    o.hi = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(pack_bf16(a2, a3)) << 32) | pack_bf16(a0, a1));
+    return o;
+}
+
+__device__ __forceinline__ floatx4 mm16_exact_a(const Opnd& A, const Opnd& B, floatx4 acc) {
+    const shortx4 ahi = __builtin_bit_cast(shortx4, A.hi);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.hi), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.lo), acc, 0, 0, 0);
     return acc;
 }
 

@@ -263,22 +263,33 @@ void fused_kernel(LaunchArgs a) {
         }
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
+        // Split BF16 with 8-bit samples and a separable window: the samples are exact in
+        // BF16, so the first contraction takes them unscaled (two MFMA, no split) and the
+        // slice scale as[t] multiplies D2 instead.
+        constexpr bool kExactIn = PREC == 1 && SEP && sizeof(Tin) == 1;
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            float wv[4];
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-                if constexpr (SEP) {
-                    wv[q] = as[t] * cur[t][q];
-                } else {
-                    wv[q] = win[t][q] * cur[t][q];
-                }
-            }
-            const Opnd in = make_opnd<PREC>(wv[0], wv[1], wv[2], wv[3]);
             const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-            const floatx4 d1 = mm16<PREC>(in, WXo, zero);
+            floatx4 d1;
+            if constexpr (kExactIn) {
+                d1 = mm16_exact_a(make_opnd_exact(cur[t][0], cur[t][1], cur[t][2], cur[t][3]), WXo, zero);
+            } else {
+                float wv[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if constexpr (SEP) {
+                        wv[q] = as[t] * cur[t][q];
+                    } else {
+                        wv[q] = win[t][q] * cur[t][q];
+                    }
+                }
+                d1 = mm16<PREC>(make_opnd<PREC>(wv[0], wv[1], wv[2], wv[3]), WXo, zero);
+            }
             d2[t] = mm16<PREC>(WYo, make_opnd<PREC>(d1[0], d1[1], d1[2], d1[3]), zero);
+            if constexpr (kExactIn) {
+                d2[t] *= as[t];
+            }
         }
 
         // ---- temporal real basis, staged as [j][p][q][b*2+c] ----
