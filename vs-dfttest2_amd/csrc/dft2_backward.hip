@@ -282,31 +282,32 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             d2[t] = mm16<kBP>(WYo, make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
         {
-            // temporal basis from the sums / differences of slices t and T - t
+            // centre-indexed temporal basis from the sums / differences of slices R + k
+            // and R - k (as in the forward kernel)
             floatx4 tsum[R + 1], tdif[R + 1];
 #pragma unroll
-            for (int t = 1; t <= R; t++) {
-                tsum[t] = d2[t] + d2[T - t];
-                tdif[t] = d2[t] - d2[T - t];
+            for (int k = 1; k <= R; k++) {
+                tsum[k] = d2[R + k] + d2[R - k];
+                tdif[k] = d2[R + k] - d2[R - k];
             }
 #pragma unroll
             for (int j = 0; j < T; j++) {
-                floatx4 u = d2[0];
+                floatx4 u = d2[R];
                 if (j == 0) {
 #pragma unroll
-                    for (int t = 1; t <= R; t++) {
-                        u += tsum[t];
+                    for (int k = 1; k <= R; k++) {
+                        u += tsum[k];
                     }
                 } else if (j <= R) {
 #pragma unroll
-                    for (int t = 1; t <= R; t++) {
-                        u += tsum[t] * wt[j][t];
+                    for (int k = 1; k <= R; k++) {
+                        u += tsum[k] * wt[j][k];
                     }
                 } else if constexpr (R > 0) {
                     u = tdif[1] * wt[j][1];
 #pragma unroll
-                    for (int t = 2; t <= R; t++) {
-                        u += tdif[t] * wt[j][t];
+                    for (int k = 2; k <= R; k++) {
+                        u += tdif[k] * wt[j][k];
                     }
                 }
 #pragma unroll
@@ -377,19 +378,19 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
                 br[e] = ex ? z0r - zxi : zxr;
                 bi[e] = ex ? -z0i - zxr : zxi;
             }
-            // dV -> d(real t,y basis) -> dP, dQ per x slot
+            // dV -> d(real t,y basis) -> dP, dQ per x slot; at the centre slice only the
+            // cos-t components carry weight (the sin-t rows vanish there)
             const float wcg = ct[g];
-            const float wsg = g ? ct[R + g] : 0.0f;
             const float dv0[2] = {dV4.x, dV4.y}, dv1[2] = {dV4.z, dV4.w};
             float dzr[2][2], dzi[2][2]; // [e][c] gradients of the gained Z'
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                const float duc0 = wcg * dv0[c], dus0 = wsg * dv0[c];
-                const float duc1 = wcg * dv1[c], dus1 = wsg * dv1[c];
-                dzr[0][c] = ey ? duc0 - dus1 : duc0;
-                dzr[1][c] = ey ? duc0 + dus1 : duc1;
-                dzi[0][c] = ey ? -(dus0 + duc1) : -dus0;
-                dzi[1][c] = ey ? duc1 - dus0 : -dus1;
+                const float duc0 = wcg * dv0[c];
+                const float duc1 = wcg * dv1[c];
+                dzr[0][c] = duc0;
+                dzr[1][c] = ey ? duc0 : duc1;
+                dzi[0][c] = ey ? -duc1 : 0.0f;
+                dzi[1][c] = ey ? duc1 : 0.0f;
             }
             float gz1r[2], gz1i[2], gz2r[2], gz2i[2]; // gradients of Z (pre gain), [c]
 #pragma unroll
@@ -446,29 +447,29 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
                 du[j][i] = wbuf[((j * 8 + (m & 7)) * 8 + wq) * 4 + (m >> 3) * 2 + wc];
             }
         }
-        // adjoint temporal basis: slice t gets C_t + S_t and slice T - t gets C_t - S_t
-        // (C: cos rows, even in t; S: sin rows, odd in t)
+        // adjoint temporal basis: slice R + k gets C_k + S_k and slice R - k gets C_k - S_k
+        // (C: cos rows, even in k; S: sin rows, odd in k); the centre slice the cos rows
         floatx4 ddt[T];
-        ddt[0] = du[0];
+        ddt[R] = du[0];
 #pragma unroll
         for (int j = 1; j <= R; j++) {
-            ddt[0] += du[j];
+            ddt[R] += du[j];
         }
         if constexpr (R > 0) {
 #pragma unroll
-            for (int t = 1; t <= R; t++) {
+            for (int k = 1; k <= R; k++) {
                 floatx4 c = du[0];
 #pragma unroll
                 for (int j = 1; j <= R; j++) {
-                    c += du[j] * wt[j][t];
+                    c += du[j] * wt[j][k];
                 }
-                floatx4 sn = du[R + 1] * wt[R + 1][t];
+                floatx4 sn = du[R + 1] * wt[R + 1][k];
 #pragma unroll
-                for (int k = 2; k <= R; k++) {
-                    sn += du[R + k] * wt[R + k][t];
+                for (int i = 2; i <= R; i++) {
+                    sn += du[R + i] * wt[R + i][k];
                 }
-                ddt[t] = c + sn;
-                ddt[T - t] = c - sn;
+                ddt[R + k] = c + sn;
+                ddt[R - k] = c - sn;
             }
         }
         wave_lds_sync(); // du read before the staged outputs overwrite the buffer

@@ -130,8 +130,8 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
             double v = j <= r ? std::cos(2 * pi * j * t / T) : std::sin(2 * pi * (j - r) * t / T);
             c[kOffWT + j * T + t] = static_cast<float>(v);
         }
-        double vr = j <= r ? std::cos(2 * pi * j * r / T) : std::sin(2 * pi * (j - r) * r / T);
-        c[kOffCT + j] = static_cast<float>((j == 0 ? 1.0 : 2.0) * vr);
+        // centre slice t' = 0: cos rows 1, sin rows 0
+        c[kOffCT + j] = static_cast<float>(j == 0 ? 1.0 : (j <= r ? 2.0 : 0.0));
     }
     auto sigma_at = [&](int kt, int ky, int kx) -> float {
         if (prm->sigma_len == 1) {
@@ -139,11 +139,15 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
         }
         return static_cast<float>(prm->sigma[(kt * 16 + ky) * 9 + kx]);
     };
+    // Window spectrum in the centre-indexed temporal convention: times exp(2 pi i kt r / T).
     auto wf_at = [&](int kt, int ky, int kx, int part) -> float {
         if (!p->zero_mean) {
             return 0.0f;
         }
-        return static_cast<float>(prm->window_freq[((kt * 16 + ky) * 9 + kx) * 2 + part]);
+        const double re = prm->window_freq[((kt * 16 + ky) * 9 + kx) * 2];
+        const double im = prm->window_freq[((kt * 16 + ky) * 9 + kx) * 2 + 1];
+        const double ph = 2 * pi * kt * r / T;
+        return static_cast<float>(part == 0 ? re * std::cos(ph) - im * std::sin(ph) : re * std::sin(ph) + im * std::cos(ph));
     };
     // Middle-stage ownership (see dft2_kernels.hip): lane = 8p + q, evaluations e = 0, 1,
     // slots A (2e) and B (2e+1).
@@ -175,7 +179,8 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
             }
         }
     }
-    // Separable window: w[t][y][x] = a[t] b[y] c[x] to within 1e-9 of its peak.
+    // Separable window: w[t][y][x] = a[t] b[y] c[x] to within 1e-9 of its peak, with a[t]
+    // symmetric about the centre slice (every DFTTest window type is).
     {
         const double* w = prm->window;
         int t0 = 0, y0 = 0, x0 = 0;
@@ -198,6 +203,11 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
                 cx[i] = w[(t0 * 16 + y0) * 16 + i] / w0;
             }
             bool sep = true;
+            for (int k = 1; k <= r; k++) {
+                if (std::fabs(at[r + k] - at[r - k]) > 1e-9 * peak) {
+                    sep = false;
+                }
+            }
             for (int t = 0; t < T && sep; t++) {
                 for (int y = 0; y < 16 && sep; y++) {
                     for (int x = 0; x < 16; x++) {
@@ -221,8 +231,11 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
                     c[kOffAS + t] = static_cast<float>(static_cast<double>(scale) * at[t]);
                 }
                 for (int j = 0; j < T; j++) {
-                    const double vr = j <= r ? std::cos(2 * pi * j * r / T) : std::sin(2 * pi * (j - r) * r / T);
-                    c[kOffCTS + j] = static_cast<float>((j == 0 ? 1.0 : 2.0) * vr * at[r]);
+                    c[kOffCTS + j] = static_cast<float>((j == 0 ? 1.0 : (j <= r ? 2.0 : 0.0)) * at[r]);
+                    for (int k = 1; k <= r; k++) {
+                        const double v = j <= r ? std::cos(2 * pi * j * k / T) : std::sin(2 * pi * (j - r) * k / T);
+                        c[kOffWTS + j * T + k] = static_cast<float>(v * static_cast<double>(scale) * at[r + k]);
+                    }
                 }
             }
         }

@@ -113,11 +113,9 @@ void fused_kernel(LaunchArgs a) {
             }
         }
     }
-    float as[T]; // SEP: sample scale * temporal window factor of slice t (uniform)
-#pragma unroll
-    for (int t = 0; t < T; t++) {
-        as[t] = SEP ? a.consts[kOffAS + t] : 1.0f;
-    }
+    // SEP: sample scale * temporal window factor of the centre slice; the other slices'
+    // factors are folded into the temporal basis (WTS).
+    const float as_c = SEP ? a.consts[kOffAS + R] : 1.0f;
     const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
     const Opnd WYo = make_opnd<PREC>(wy[0], wy[1], wy[2], wy[3]);
     const Opnd GXo = make_opnd<PREC>(gx[0], gx[1], gx[2], gx[3]);
@@ -129,7 +127,7 @@ void fused_kernel(LaunchArgs a) {
         ct[j] = a.consts[(SEP ? kOffCTS : kOffCT) + j];
 #pragma unroll
         for (int t = 0; t < T; t++) {
-            wt[j][t] = a.consts[kOffWT + j * T + t];
+            wt[j][t] = a.consts[(SEP ? kOffWTS : kOffWT) + j * T + t];
         }
     }
 
@@ -263,9 +261,9 @@ void fused_kernel(LaunchArgs a) {
         }
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
-        // Split BF16 with 8-bit samples and a separable window: the samples are exact in
-        // BF16, so the first contraction takes them unscaled (two MFMA, no split) and the
-        // slice scale as[t] multiplies D2 instead.
+        // Separable window: the slices enter unscaled (their factors are in WTS and as_c).
+        // Split BF16 with 8-bit samples: the samples are exact in BF16, so the first
+        // contraction takes them as they are (two MFMA, no split).
         constexpr bool kExactIn = PREC == 1 && SEP && sizeof(Tin) == 1;
         floatx4 d2[T];
 #pragma unroll
@@ -278,49 +276,43 @@ void fused_kernel(LaunchArgs a) {
                 float wv[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    if constexpr (SEP) {
-                        wv[q] = as[t] * cur[t][q];
-                    } else {
-                        wv[q] = win[t][q] * cur[t][q];
-                    }
+                    wv[q] = SEP ? cur[t][q] : win[t][q] * cur[t][q];
                 }
                 d1 = mm16<PREC>(make_opnd<PREC>(wv[0], wv[1], wv[2], wv[3]), WXo, zero);
             }
             d2[t] = mm16<PREC>(WYo, make_opnd<PREC>(d1[0], d1[1], d1[2], d1[3]), zero);
-            if constexpr (kExactIn) {
-                d2[t] *= as[t];
-            }
         }
 
-        // ---- temporal real basis, staged as [j][p][q][b*2+c] ----
-        // cos(2 pi j t/T) is even and sin odd under t -> T - t, so the rows need only
-        // the sums and differences of the slice pairs (t, T - t).
-        floatx4 tsum[R + 1], tdif[R + 1]; // [t] for t = 1..R
+        // ---- temporal real basis (centre-indexed), staged as [j][p][q][b*2+c] ----
+        // cos(2 pi j k/T) is even and sin odd in k = t - R, so the rows need only the
+        // sums and differences of the slice pairs (R + k, R - k).
+        floatx4 tsum[R + 1], tdif[R + 1]; // [k] for k = 1..R
 #pragma unroll
-        for (int t = 1; t <= R; t++) {
-            tsum[t] = d2[t] + d2[T - t];
-            tdif[t] = d2[t] - d2[T - t];
+        for (int k = 1; k <= R; k++) {
+            tsum[k] = d2[R + k] + d2[R - k];
+            tdif[k] = d2[R + k] - d2[R - k];
         }
+        const floatx4 dc = SEP ? d2[R] * as_c : d2[R];
 #pragma unroll
         for (int j = 0; j < T; j++) {
             floatx4 u;
             if (j == 0) {
-                u = d2[0];
+                u = dc;
 #pragma unroll
-                for (int t = 1; t <= R; t++) {
-                    u += tsum[t];
+                for (int k = 1; k <= R; k++) {
+                    u += SEP ? tsum[k] * wt[0][k] : tsum[k];
                 }
             } else if (j <= R) {
-                u = d2[0];
+                u = dc;
 #pragma unroll
-                for (int t = 1; t <= R; t++) {
-                    u += tsum[t] * wt[j][t];
+                for (int k = 1; k <= R; k++) {
+                    u += tsum[k] * wt[j][k];
                 }
             } else if constexpr (R > 0) {
                 u = tdif[1] * wt[j][1];
 #pragma unroll
-                for (int t = 2; t <= R; t++) {
-                    u += tdif[t] * wt[j][t];
+                for (int k = 2; k <= R; k++) {
+                    u += tdif[k] * wt[j][k];
                 }
             }
 #pragma unroll
@@ -386,19 +378,17 @@ void fused_kernel(LaunchArgs a) {
                 zi[e][1] = ex ? ar - br : bi;
             }
 
-            // Back to the real (t, y) basis, then the centre-slice inverse weights.
+            // Back to the real (t, y) basis at the centre slice, where the sin-t rows
+            // vanish: only the cos-t components, times the centre weights.
             const float wc_ = ct[g];
-            const float ws_ = g ? ct[R + g] : 0.0f;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 const float pr = zr[0][c], pi = zi[0][c], qr = zr[1][c], qi = zi[1][c];
                 // doubled for a proper y pair (see gy above)
-                const float u_c0 = ey ? pr + qr : pr;     // cos t, y slot 0
-                const float u_s1 = ey ? qr - pr : -qi;    // sin t, y slot 1
-                const float u_s0 = ey ? -(pi + qi) : -pi; // sin t, y slot 0
-                const float u_c1 = ey ? qi - pi : qr;     // cos t, y slot 1
-                const float y0 = wc_ * u_c0 + ws_ * u_s0;
-                const float y1 = wc_ * u_c1 + ws_ * u_s1;
+                const float u_c0 = ey ? pr + qr : pr; // cos t, y slot 0
+                const float u_c1 = ey ? qi - pi : qr; // cos t, y slot 1
+                const float y0 = wc_ * u_c0;
+                const float y1 = wc_ * u_c1;
                 if (c == 0) {
                     v00 += y0;
                     v10 += y1;

@@ -18,10 +18,17 @@ constexpr int kThreads = kWaves * 64;
 //   GY [y][m]   inverse y transform, WY^T with the same weights
 //   WIN [t][y][x] analysis window times the sample scale (float32, as the plugin stores it)
 //   SYN [y][x]  synthesis window (window slice t = radius)
-//   WT [j][t]   temporal real basis: j = 0..r cos(2 pi j t/T), j = r+1..2r sin(2 pi (j-r) t/T)
-//   CT [j]      centre-slice inverse weights: WT[j][r] * (1 for j = 0, else 2)
+//   The temporal transform is indexed from the centre slice, t' = t - r (t' = -r..r): its
+//   basis functions are even (cos) or odd (sin) in t', slices r + k and r - k pair up,
+//   and at the centre slice every sin row vanishes.  A coefficient differs from the
+//   reference's (t = 0..T-1) by the phase exp(2 pi i kt r / T) only, which leaves every
+//   gain unchanged; the window spectrum in TAB carries the same phase.
+//   WT [j][k]   temporal real basis at t' = k (k = 1..r used): j = 0..r cos(2 pi j k/T),
+//               j = r+1..2r sin(2 pi (j-r) k/T)
+//   CT [j]      centre-slice inverse weights: 1 (j = 0), 2 (j = 1..r), 0 (sin rows)
 //   TAB [g][slot][lane][4]  per-lane gain table of the middle stage: sigma, partner sigma,
-//               Re and Im of window_freq at the coefficient the lane owns (see kernel).
+//               Re and Im of window_freq at the coefficient the lane owns (see kernel),
+//               times exp(2 pi i kt r / T).
 constexpr int kOffWX = 0;
 constexpr int kOffWY = 256;
 constexpr int kOffGX = 512;
@@ -36,13 +43,16 @@ constexpr int kTabFloats(int radius) { return (radius + 1) * 4 * 64 * 4; }
 // factors fold into the bases, so the kernel needs no per-lane window values.
 //   SWX [x][n] = WX[x][n] c[x]    (forward x operand; read transposed: inverse x operand)
 //   SWY [m][y] = WY[m][y] b[y]    (forward y operand; read transposed: inverse y operand)
-//   AS  [t]    = sample scale * a[t]   (per-slice factor of the analysis window)
+//   AS  [t]    = sample scale * a[t]   (per-slice factor of the analysis window; a[t] is
+//                symmetric about the centre, a[r + k] = a[r - k], for the separable form)
 //   CTS [j]    = CT[j] * a[r]          (centre weights with the synthesis window's a[r])
+//   WTS [j][k] = WT[j][k] * AS[r + k]  (temporal basis with the slice factors folded in)
 constexpr int kOffSWX = kOffTAB + 4 * 4 * 64 * 4;
 constexpr int kOffSWY = kOffSWX + 256;
 constexpr int kOffAS = kOffSWY + 256;         // up to 7, padded to 8
 constexpr int kOffCTS = kOffAS + 8;           // up to 7, padded to 8
-constexpr int kConstFloats = kOffCTS + 8;
+constexpr int kOffWTS = kOffCTS + 8;          // up to 49, padded to 64
+constexpr int kConstFloats = kOffWTS + 64;
 
 // One plane of one clip in a launch (device copy of dft2_job plus derived geometry).
 struct JobDev {
