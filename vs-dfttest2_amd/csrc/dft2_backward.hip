@@ -188,6 +188,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
     const int mq = lane & 7;
     const bool ey = mp != 0;
     const bool ex = mq != 0;
+    // 1/0 factors: "c ? a + b : a" is fma(cf, b, a) (one instruction, same value)
+    const float eyf = ey ? 1.0f : 0.0f;
+    const float exf = ex ? 1.0f : 0.0f;
     const int wq = l16 & 7;
     const int wc = l16 >> 3;
     float2 dsg[R + 1][4];
@@ -360,8 +363,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             float z1r[2], z1i[2], z2r[2], z2i[2];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                z1r[c] = cc0[c] - (ey ? ss1[c] : 0.0f);
-                z1i[c] = -(ss0[c] + (ey ? cc1[c] : 0.0f));
+                z1r[c] = fmaf(-eyf, ss1[c], cc0[c]);
+                z1i[c] = -fmaf(eyf, cc1[c], ss0[c]);
                 z2r[c] = ey ? (cc0[c] + ss1[c]) : cc1[c];
                 z2i[c] = ey ? (cc1[c] - ss0[c]) : -ss1[c];
             }
@@ -373,8 +376,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
                 const float z0i = e ? z2i[0] : z1i[0];
                 const float zxr = e ? z2r[1] : z1r[1];
                 const float zxi = e ? z2i[1] : z1i[1];
-                ar[e] = ex ? z0r + zxi : z0r;
-                ai[e] = ex ? z0i - zxr : z0i;
+                ar[e] = fmaf(exf, zxi, z0r);
+                ai[e] = fmaf(-exf, zxr, z0i);
                 br[e] = ex ? z0r - zxi : zxr;
                 bi[e] = ex ? -z0i - zxr : zxi;
             }
@@ -396,15 +399,15 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
 #pragma unroll
             for (int e = 0; e < 2; e++) {
                 const float d0r = dzr[e][0], d0i = dzi[e][0], d1r = dzr[e][1], d1i = dzi[e][1];
-                float dar = ex ? d0r + d1i : d0r;
-                float dai = ex ? d0i - d1r : d0i;
+                float dar = fmaf(exf, d1i, d0r);
+                float dai = fmaf(-exf, d1r, d0i);
                 float dbr = ex ? d0r - d1i : d1r;
                 float dbi = ex ? -d0i - d1r : d1i;
                 gain_adjoint(ar[e], ai[e], dar, dai, tab[(g * 4 + 2 * e) * 64 + lane], gf, dsg[g][2 * e], dgf);
                 gain_adjoint(br[e], bi[e], dbr, dbi, tab[(g * 4 + 2 * e + 1) * 64 + lane], gf, dsg[g][2 * e + 1],
                              dgf);
-                const float g0r = ex ? dar + dbr : dar;
-                const float g0i = ex ? dai - dbi : dai;
+                const float g0r = fmaf(exf, dbr, dar);
+                const float g0i = fmaf(-exf, dbi, dai);
                 const float g1r = ex ? -dai - dbi : dbr;
                 const float g1i = ex ? dar - dbr : dbi;
                 if (e == 0) {
@@ -416,9 +419,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             float dc0[2], dc1[2], ds0[2], ds1[2];
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                dc0[c] = ey ? gz1r[c] + gz2r[c] : gz1r[c];
+                dc0[c] = fmaf(eyf, gz2r[c], gz1r[c]);
                 ds1[c] = ey ? gz2r[c] - gz1r[c] : -gz2i[c];
-                ds0[c] = ey ? -gz1i[c] - gz2i[c] : -gz1i[c];
+                ds0[c] = -fmaf(eyf, gz2i[c], gz1i[c]);
                 dc1[c] = ey ? gz2i[c] - gz1i[c] : gz2r[c];
             }
             dU4[g] = make_float4(dc0[0], dc0[1], dc1[0], dc1[1]);

@@ -138,6 +138,10 @@ void fused_kernel(LaunchArgs a) {
     const int mq = lane & 7;
     const bool ey = mp != 0;
     const bool ex = mq != 0;
+    // The same as 1/0 factors: "c ? a + b : a" is fma(cf, b, a), one instruction and the
+    // same value for finite b.
+    const float eyf = ey ? 1.0f : 0.0f;
+    const float exf = ex ? 1.0f : 0.0f;
     const int wq = l16 & 7;
     const int wc = l16 >> 3;
 
@@ -349,8 +353,8 @@ void fused_kernel(LaunchArgs a) {
                 const float ss0[2] = {s00, s01}, ss1[2] = {s10, s11};
 #pragma unroll
                 for (int c = 0; c < 2; c++) {
-                    z1r[c] = cc0[c] - (ey ? ss1[c] : 0.0f);
-                    z1i[c] = -(ss0[c] + (ey ? cc1[c] : 0.0f));
+                    z1r[c] = fmaf(-eyf, ss1[c], cc0[c]);
+                    z1i[c] = -fmaf(eyf, cc1[c], ss0[c]);
                     z2r[c] = ey ? (cc0[c] + ss1[c]) : cc1[c];
                     z2i[c] = ey ? (cc1[c] - ss0[c]) : -ss1[c];
                 }
@@ -365,15 +369,15 @@ void fused_kernel(LaunchArgs a) {
                 const float zxi = e ? z2i[1] : z1i[1];
                 // proper x pair: A = Z0 - i Z1 (+kx), B = conj(Z0) - i conj(Z1) (conjugate
                 // partner side); pair 0: A = column kx=0, B = column kx=8.
-                float ar = ex ? z0r + zxi : z0r;
-                float ai = ex ? z0i - zxr : z0i;
+                float ar = fmaf(exf, zxi, z0r);
+                float ai = fmaf(-exf, zxr, z0i);
                 float br = ex ? z0r - zxi : zxr;
                 float bi = ex ? -z0i - zxr : zxi;
                 filter_coef<FTM, ASYM>(ar, ai, tab_entry<ASYM>(&tab[(g * 4 + 2 * e) * 64 + lane]), gf, a);
                 filter_coef<FTM, ASYM>(br, bi, tab_entry<ASYM>(&tab[(g * 4 + 2 * e + 1) * 64 + lane]), gf, a);
                 // x slots back from (A, B), doubled for a proper pair (see gx above).
-                zr[e][0] = ex ? ar + br : ar;
-                zi[e][0] = ex ? ai - bi : ai;
+                zr[e][0] = fmaf(exf, br, ar);
+                zi[e][0] = fmaf(-exf, bi, ai);
                 zr[e][1] = ex ? -(ai + bi) : br;
                 zi[e][1] = ex ? ar - br : bi;
             }
@@ -385,7 +389,7 @@ void fused_kernel(LaunchArgs a) {
             for (int c = 0; c < 2; c++) {
                 const float pr = zr[0][c], pi = zi[0][c], qr = zr[1][c], qi = zi[1][c];
                 // doubled for a proper y pair (see gy above)
-                const float u_c0 = ey ? pr + qr : pr; // cos t, y slot 0
+                const float u_c0 = fmaf(eyf, qr, pr); // cos t, y slot 0 (pr + qr for a proper pair)
                 const float u_c1 = ey ? qi - pi : qr; // cos t, y slot 1
                 const float y0 = wc_ * u_c0;
                 const float y1 = wc_ * u_c1;
