@@ -501,13 +501,18 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             const int Sm = (65536 + Sg - 1) / Sg; // for div_small
             const int nb = min(kWaves, nbx - pass * kWaves);
             const int cols = Sg * (nb - 1) + 16;
+            for (int x0 = 0; x0 < cols; x0 += 64) {
+                const int x = x0 + lane;
+                if (x >= cols) {
+                    continue;
+                }
+                // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S); the
+                // bounds and masks serve the wave's rows y = wave, wave + 8 of every slice
+                const int lo = div_small(max(x + Sg - 16, 0), Sm);
+                const int hi = min(div_small(x, Sm), nb - 1);
 #pragma unroll
-            for (int yy = 0; yy < 16; yy += kWaves) {
-                const int y = yy + wave;
-                for (int x = lane; x < cols; x += 64) {
-                    // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)
-                    const int lo = x >= 15 ? div_small(x - 15 + Sg - 1, Sm) : 0;
-                    const int hi = min(div_small(x, Sm), nb - 1);
+                for (int yy = 0; yy < 16; yy += kWaves) {
+                    const int y = yy + wave;
                     const int g = (r * Sg + y) * rw + pass * kWaves * Sg + x;
 #pragma unroll
                     for (int t = 0; t < T; t++) {

@@ -122,9 +122,10 @@ __device__ __forceinline__ int opaque_uniform(int v) {
 }
 
 // x / S for 0 <= x < 256 and 1 <= S <= 16, given m = ceil(65536 / S): exact, since
-// x * m / 65536 exceeds x / S by less than 256 / 65536 < 1 / S.
+// x * m / 65536 exceeds x / S by less than 256 / 65536 < 1 / S.  x * m < 2^24, so one
+// 24-bit multiply.
 __device__ __forceinline__ int div_small(int x, int m) {
-    return (x * m) >> 16;
+    return static_cast<int>(__umul24(static_cast<unsigned>(x), static_cast<unsigned>(m)) >> 16);
 }
 
 // Overlap-add gather: the sum of the staged values of blocks lo..hi covering one pixel,
@@ -135,7 +136,7 @@ __device__ __forceinline__ int div_small(int x, int m) {
 template <int kStride>
 __device__ __forceinline__ float gather_sum(const float* p, int lo, int hi) {
     const int k0 = max(hi - 3, 0);
-    const float* q = p + k0 * kStride;
+    const float* q = p + __umul24(static_cast<unsigned>(k0), static_cast<unsigned>(kStride));
     float sum = 0.0f;
 #pragma unroll
     for (int j = 0; j < 4; j++) {

@@ -450,16 +450,19 @@ void fused_kernel(LaunchArgs a) {
             const int Sm = (65536 + Sg - 1) / Sg; // for div_small
             const int nb = min(kWaves, nbx - pass * kWaves);
             const int cols = Sg * (nb - 1) + 16;
+            for (int x0 = 0; x0 < cols; x0 += 64) {
+                const int x = x0 + lane;
+                if (x >= cols) {
+                    continue;
+                }
+                // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S); the
+                // bounds and masks serve the wave's rows y = wave, wave + 8
+                const int lo = div_small(max(x + Sg - 16, 0), Sm);
+                const int hi = min(div_small(x, Sm), nb - 1);
+                float* acol = acc + (r * Sg + wave) * rw + pass * kWaves * Sg + x;
 #pragma unroll
-            for (int yy = 0; yy < 16; yy += kWaves) {
-                const int y = yy + wave;
-                const float* sy = stage + y * kOut;
-                float* arow = acc + (r * Sg + y) * rw + pass * kWaves * Sg;
-                for (int x = lane; x < cols; x += 64) {
-                    // column x is covered by blocks ceil((x - 15) / S) .. floor(x / S)
-                    const int lo = x >= 15 ? div_small(x - 15 + Sg - 1, Sm) : 0;
-                    const int hi = min(div_small(x, Sm), nb - 1);
-                    arow[x] += gather_sum<kWbuf>(sy + x, lo, hi);
+                for (int yy = 0; yy < 16; yy += kWaves) {
+                    acol[yy * rw] += gather_sum<kWbuf>(stage + (yy + wave) * kOut + x, lo, hi);
                 }
             }
         }
