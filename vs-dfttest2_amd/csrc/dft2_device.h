@@ -104,6 +104,40 @@ __device__ __forceinline__ floatx4 mm16_exact_a(const Opnd& A, const Opnd& B, fl
     return acc;
 }
 
+// FP32 contraction whose A operand is exact in BF16, on the BF16 MFMA.  An FP32 value is
+// the exact sum of three BF16 values (hi = its rounding to 8 significant bits, then the
+// same for the remainder, and the remainder of that, which has at most 8 bits left), so
+// A*B = A*Bhi + A*Bmid + A*Blo with every product exact (8 x 8 significant bits) and the
+// sums accumulated in FP32: the accuracy of an FP32 contraction, in three 16x16x16 BF16
+// MFMA instead of four 16x16x4 FP32 MFMA.  Smallest part first.
+struct Opnd3 {
+    bf16x4 hi, mid, lo;
+};
+
+__device__ __forceinline__ Opnd3 make_opnd3(float a0, float a1, float a2, float a3) {
+    float h0, h1, h2, h3, m0, m1, m2, m3;
+    const unsigned hi01 = pack_hi(a0, a1, h0, h1);
+    const unsigned hi23 = pack_hi(a2, a3, h2, h3);
+    const float r0 = a0 - h0, r1 = a1 - h1, r2 = a2 - h2, r3 = a3 - h3;
+    const unsigned mid01 = pack_hi(r0, r1, m0, m1);
+    const unsigned mid23 = pack_hi(r2, r3, m2, m3);
+    const unsigned lo01 = pack_bf16(r0 - m0, r1 - m1);
+    const unsigned lo23 = pack_bf16(r2 - m2, r3 - m3);
+    Opnd3 o;
+    o.hi = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(hi23) << 32) | hi01);
+    o.mid = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(mid23) << 32) | mid01);
+    o.lo = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(lo23) << 32) | lo01);
+    return o;
+}
+
+__device__ __forceinline__ floatx4 mm16_exact3(const Opnd& A, const Opnd3& B, floatx4 acc) {
+    const shortx4 ahi = __builtin_bit_cast(shortx4, A.hi);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.lo), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.mid), acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.hi), acc, 0, 0, 0);
+    return acc;
+}
+
 // Orders this wave's LDS accesses before and after this point, so that values one lane
 // writes can be read by another lane of the same wave afterwards.  The LDS executes a
 // wave's instructions in issue order, so only the compiler has to be kept from moving

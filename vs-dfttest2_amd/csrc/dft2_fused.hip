@@ -117,6 +117,7 @@ void fused_kernel(LaunchArgs a) {
     // factors are folded into the temporal basis (WTS).
     const float as_c = SEP ? a.consts[kOffAS + R] : 1.0f;
     const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
+    const Opnd3 WX3 = make_opnd3(wx[0], wx[1], wx[2], wx[3]); // FP32 with 8-bit samples
     const Opnd WYo = make_opnd<PREC>(wy[0], wy[1], wy[2], wy[3]);
     const Opnd GXo = make_opnd<PREC>(gx[0], gx[1], gx[2], gx[3]);
     const Opnd GYo = make_opnd<PREC>(gy[0], gy[1], gy[2], gy[3]);
@@ -266,16 +267,19 @@ void fused_kernel(LaunchArgs a) {
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
         // Separable window: the slices enter unscaled (their factors are in WTS and as_c).
-        // Split BF16 with 8-bit samples: the samples are exact in BF16, so the first
-        // contraction takes them as they are (two MFMA, no split).
-        constexpr bool kExactIn = PREC == 1 && SEP && sizeof(Tin) == 1;
+        // 8-bit samples are exact in BF16, so the first contraction takes them as they are:
+        // split BF16, two MFMA and no split; FP32, the BF16 MFMA over the exact three-part
+        // split of WX (three MFMA, every product exact, FP32 accumulation).
+        constexpr bool kExactIn = SEP && sizeof(Tin) == 1;
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
             const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
             floatx4 d1;
-            if constexpr (kExactIn) {
+            if constexpr (kExactIn && PREC == 1) {
                 d1 = mm16_exact_a(make_opnd_exact(cur[t][0], cur[t][1], cur[t][2], cur[t][3]), WXo, zero);
+            } else if constexpr (kExactIn) {
+                d1 = mm16_exact3(make_opnd_exact(cur[t][0], cur[t][1], cur[t][2], cur[t][3]), WX3, zero);
             } else {
                 float wv[4];
 #pragma unroll
