@@ -88,6 +88,29 @@ def test_random_params_vs_oracle(fmt, shape, kw):
     assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= tol(is_float)
 
 
+@pytest.mark.parametrize("precision", ["fp32", "split_bf16"])
+@pytest.mark.parametrize("pattern", ["checker", "binary_noise", "impulses"])
+def test_full_scale_8bit_content(pattern, precision):
+    # With 8-bit samples both precisions take the samples into the first contraction as
+    # exact BF16 values (FP32: against the exact three-part split of the basis); full-scale
+    # content gives the spectra their widest dynamic range.
+    announce("full-scale u8", pattern, precision)
+    n, h, w = 5, 52, 68
+    rng = np.random.default_rng(11)
+    if pattern == "checker":
+        yy, xx = np.indices((h, w))
+        x = np.stack([((yy + xx + t) % 2) * 255 for t in range(n)])
+    elif pattern == "binary_noise":
+        x = rng.integers(0, 2, (n, h, w)) * 255
+    else:
+        x = np.zeros((n, h, w), np.int64)
+        x.reshape(-1)[rng.choice(x.size, 60, replace=False)] = 255
+    x = x.astype(np.uint8)
+    p = dfttest_params()
+    out = gpu_filter(x, p, 8, False, Backend.MI355X(precision=precision))
+    assert max_err(out, onp.process_plane(x, p, 8, False)) <= 1
+
+
 def test_asymmetric_sigma_array():
     announce("asymmetric sigma")
     p = dfttest_params()
