@@ -52,7 +52,9 @@ extern "C" {
 #define DFT2_SAMPLE_FLOAT 1
 
 /* Precision of the DFT contractions. */
-#define DFT2_PRECISION_FP32 0      /* FP32 MFMA */
+#define DFT2_PRECISION_FP32 0      /* FP32 accuracy: FP32 MFMA; with 8-bit samples the x
+                                    * transform runs on the BF16 MFMA over an exact
+                                    * three-part split of its basis (exact products) */
 #define DFT2_PRECISION_SPLIT_BF16 1 /* hi+lo BF16 operands, three products, FP32 accumulate */
 
 /* Stream argument naming the null (default) stream: the work is queued there and not

@@ -6,10 +6,11 @@
 //      holds row y = lane%16, columns 4*(lane/16)..+3), applying the mirror padding and
 //      temporal clamp in the address calculation (no padded copy is ever made) and the
 //      analysis window (cpu_source/getframe_impl.cpp:105-156);
-//   2. runs the separable forward real DFT as two chained FP32 MFMA contractions per
-//      slice, D2 = WY * (slice * WX), in a real cos/sin basis (16 real values per axis);
-//      the output of the first MFMA is already the B operand of the second, so no
-//      transpose is needed;
+//   2. runs the separable forward real DFT as two chained MFMA contractions per slice,
+//      D2 = WY * (slice * WX), in a real cos/sin basis (16 real values per axis); the
+//      output of the first MFMA is already the B operand of the second, so no transpose
+//      is needed.  FP32 MFMA, except that 8-bit samples, exact in BF16, meet the exact
+//      three-part BF16 split of WX on the BF16 MFMA (exact products, FP32 sums);
 //   3. forms the temporal real basis in registers, stages the T*256 values in LDS and
 //      regroups them so that each lane owns one (y pair, x pair) group: from its 4*T
 //      values it forms its 2*T complex coefficients X[kt][ky][kx], applies zero-mean,
