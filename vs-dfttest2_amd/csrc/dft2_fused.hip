@@ -48,10 +48,9 @@ __global__ __launch_bounds__(kThreads)
 __attribute__((amdgpu_waves_per_eu(R <= 1 && FTM != 2 && SEP && !PERSIST ? 6 : 1)))
 void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
-    // Overlap-add: staged block outputs summed by the pixels' owners after a barrier (radius
-    // 0 and 1), or LDS atomics with no barrier (radius 2 and 3: with two workgroups per CU
-    // the barriers cost more than the atomics; 4K, tbsize 5: 232 against 209 ms).
-    constexpr bool kStagedOla = R <= 1;
+    // Overlap-add: staged block outputs summed by the pixels' owners after a barrier, in a
+    // fixed order (LDS float atomics, the first form, cost more at every radius: 4K tbsize
+    // 5, 171.5 against 143.8 ms).
     // Per-wave LDS: the T*256 spectrum staging, reused for V (16 x 20) and for the block
     // output, staged with row pitch kOut at offset wave*S (<= 7*16 floats) for the
     // overlap-add: 16 rows * 20 + 112 <= 448 floats.  Pitch 20 puts the four lane groups'
@@ -421,27 +420,14 @@ void fused_kernel(LaunchArgs a) {
         const floatx4 f1 = mm16<PREC>(make_opnd<PREC>(v.x, v.y, v.z, v.w), GXo, zero);
         const floatx4 o = mm16<PREC>(GYo, make_opnd<PREC>(f1[0], f1[1], f1[2], f1[3]), zero);
 
-        if constexpr (kStagedOla) {
-            // ---- synthesis window, block output staged in LDS as [y][x] at offset wave*S ----
-            wave_lds_sync(); // V is read before the staged output overwrites it
-            float* srow = wbuf + wave * S + 4 * lg * kOut + l16;
+        // ---- synthesis window, block output staged in LDS as [y][x] at offset wave*S ----
+        wave_lds_sync(); // V is read before the staged output overwrites it
+        float* srow = wbuf + wave * S + 4 * lg * kOut + l16;
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                srow[i * kOut] = SEP ? o[i] : o[i] * syn[i];
-            }
-        } else {
-            // ---- synthesis window + overlap-add by LDS atomics ----
-            float* arow = acc + (r * S + 4 * lg) * rw + (pass * kWaves + wave) * S + l16;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                atomicAdd(arow + i * rw, SEP ? o[i] : o[i] * syn[i]);
-            }
-            wave_lds_sync();
+        for (int i = 0; i < 4; i++) {
+            srow[i * kOut] = SEP ? o[i] : o[i] * syn[i];
         }
     }
-        if constexpr (!kStagedOla) {
-            continue;
-        }
         // ---- overlap-add of pass `pass` of block row r, without atomics ----
         // Every pixel (y, x) of the pass's 16-row band sums the staged outputs of the
         // pass's blocks covering it; the thread that owns the pixel in this phase adds the
@@ -473,9 +459,6 @@ void fused_kernel(LaunchArgs a) {
         }
         __syncthreads(); // staged outputs consumed before the next pass reuses the buffers
     }
-    }
-    if constexpr (!kStagedOla) {
-        __syncthreads(); // every block's atomics are done before the region is read out
     }
 
     float4* out = reinterpret_cast<float4*>(a.scratch + static_cast<long long>(item) * (a.region_h * rw));
