@@ -27,6 +27,8 @@ struct Opnd {
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float floatx2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 // BF16 hi part of (a, b) as one packed register, and the FP32 values of its halves (a
 // BF16 is the upper half of an FP32 with the same value).
@@ -135,6 +137,66 @@ __device__ __forceinline__ floatx4 mm16_exact3(const Opnd& A, const Opnd3& B, fl
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.lo), acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.mid), acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.hi), acc, 0, 0, 0);
+    return acc;
+}
+
+// Contraction whose A operand is made of integers of at most 16 bits, on the 16x16x32
+// BF16 MFMA.  A sample v = 256 h + l (h, l bytes, exact in BF16) takes two of a lane's
+// eight K slots: slot s (0..3) holds l of its sample s, slot 4 + s holds h; B holds a
+// part of the basis in slots 0..3 and 256 times it (exact) in slots 4..7, so one MFMA
+// gives A * Bpart with exact products.  FP32: the three parts of make_opnd3 (smallest
+// first); split BF16: the hi and lo parts of make_opnd<1>.  Three (two) MFMA of 16
+// cycles against four FP32 MFMA of 32.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+struct OpndW {
+    bf16x8 hi, mid, lo;
+};
+
+__device__ __forceinline__ bf16x8 part_w(float b0, float b1, float b2, float b3) {
+    const u32x4 w = {pack_bf16(b0, b1), pack_bf16(b2, b3), pack_bf16(256.0f * b0, 256.0f * b1),
+                     pack_bf16(256.0f * b2, 256.0f * b3)};
+    return __builtin_bit_cast(bf16x8, w);
+}
+
+template <int PREC>
+__device__ __forceinline__ OpndW make_opndw(float a0, float a1, float a2, float a3) {
+    float h0, h1, h2, h3;
+    pack_hi(a0, a1, h0, h1);
+    pack_hi(a2, a3, h2, h3);
+    const float r0 = a0 - h0, r1 = a1 - h1, r2 = a2 - h2, r3 = a3 - h3;
+    OpndW o;
+    o.hi = part_w(h0, h1, h2, h3);
+    if constexpr (PREC == 1) {
+        // lo = bf16(a - hi), as make_opnd<1>
+        o.lo = part_w(r0, r1, r2, r3);
+    } else {
+        float m0, m1, m2, m3;
+        pack_hi(r0, r1, m0, m1);
+        pack_hi(r2, r3, m2, m3);
+        o.mid = part_w(m0, m1, m2, m3);
+        o.lo = part_w(r0 - m0, r1 - m1, r2 - m2, r3 - m3);
+    }
+    return o;
+}
+
+// The four 16-bit samples of a raw4 (two per dword) as the A operand: low bytes in slots
+// 0..3, high bytes in slots 4..7.
+__device__ __forceinline__ bf16x8 bytes_w(unsigned d0, unsigned d1) {
+    const u32x4 w = {pack_bf16(static_cast<float>(d0 & 0xffu), static_cast<float>((d0 >> 16) & 0xffu)),
+                     pack_bf16(static_cast<float>(d1 & 0xffu), static_cast<float>((d1 >> 16) & 0xffu)),
+                     pack_bf16(static_cast<float>((d0 >> 8) & 0xffu), static_cast<float>(d0 >> 24)),
+                     pack_bf16(static_cast<float>((d1 >> 8) & 0xffu), static_cast<float>(d1 >> 24))};
+    return __builtin_bit_cast(bf16x8, w);
+}
+
+template <int PREC>
+__device__ __forceinline__ floatx4 mm16_bytes(bf16x8 A, const OpndW& B, floatx4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B.lo, acc, 0, 0, 0);
+    if constexpr (PREC == 0) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B.mid, acc, 0, 0, 0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B.hi, acc, 0, 0, 0);
     return acc;
 }
 
@@ -249,8 +311,6 @@ __device__ __forceinline__ float raw_value(raw_t<Tin> v) {
 // one dword of 4 bytes, two dwords of 2 halves, or four float dwords.  They stay packed
 // until the samples are used, so a prefetch keeps its loads in flight and holds T
 // registers per lane for 8-bit samples instead of 4*T.
-typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 template <typename Tin>
 struct raw4 {

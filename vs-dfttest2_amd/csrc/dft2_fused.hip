@@ -118,6 +118,7 @@ void fused_kernel(LaunchArgs a) {
     const float as_c = SEP ? a.consts[kOffAS + R] : 1.0f;
     const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
     const Opnd3 WX3 = make_opnd3(wx[0], wx[1], wx[2], wx[3]); // FP32 with 8-bit samples
+    const OpndW WXw = make_opndw<PREC>(wx[0], wx[1], wx[2], wx[3]); // 16-bit samples
     const Opnd WYo = make_opnd<PREC>(wy[0], wy[1], wy[2], wy[3]);
     const Opnd GXo = make_opnd<PREC>(gx[0], gx[1], gx[2], gx[3]);
     const Opnd GYo = make_opnd<PREC>(gy[0], gy[1], gy[2], gy[3]);
@@ -250,9 +251,11 @@ void fused_kernel(LaunchArgs a) {
     for (int r = 0; r < nby; r++) {
     for (int pass = 0; pass < n_pass; pass++) {
     if (pass < my_pass) {
+        raw4<Tin> cur_raw[T]; // the samples as loaded (16-bit samples' exact path)
         float cur[T][4];
 #pragma unroll
         for (int t = 0; t < T; t++) {
+            cur_raw[t] = nxt[t];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 cur[t][s] = raw4_value<Tin>(nxt[t], s);
@@ -269,8 +272,10 @@ void fused_kernel(LaunchArgs a) {
         // Separable window: the slices enter unscaled (their factors are in WTS and as_c).
         // 8-bit samples are exact in BF16, so the first contraction takes them as they are:
         // split BF16, two MFMA and no split; FP32, the BF16 MFMA over the exact three-part
-        // split of WX (three MFMA, every product exact, FP32 accumulation).
+        // split of WX (three MFMA, every product exact, FP32 accumulation).  16-bit samples
+        // enter as their two bytes on the 16x16x32 BF16 MFMA (mm16_bytes).
         constexpr bool kExactIn = SEP && sizeof(Tin) == 1;
+        constexpr bool kExactIn16 = SEP && sizeof(Tin) == 2;
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
@@ -280,6 +285,8 @@ void fused_kernel(LaunchArgs a) {
                 d1 = mm16_exact_a(make_opnd_exact(cur[t][0], cur[t][1], cur[t][2], cur[t][3]), WXo, zero);
             } else if constexpr (kExactIn) {
                 d1 = mm16_exact3(make_opnd_exact(cur[t][0], cur[t][1], cur[t][2], cur[t][3]), WX3, zero);
+            } else if constexpr (kExactIn16) {
+                d1 = mm16_bytes<PREC>(bytes_w(cur_raw[t].v[0], cur_raw[t].v[1]), WXw, zero);
             } else {
                 float wv[4];
 #pragma unroll
