@@ -286,11 +286,28 @@ def test_repeated_and_alternating_job_tables():
             xs[i].mul_(0.9).add_(0.03)
             plan.process_batch([plane_job(xs[i], ys[i])])
             fresh.process_batch([plane_job(xs[i], refs[i])])
-            # overlap-add order within a tile varies between runs (LDS atomics), so
-            # float results agree to rounding, not bit for bit
-            assert torch.max(torch.abs(ys[i] - refs[i])).item() <= 1e-6, (rep, i)
+            # the overlap-add sums in a fixed order: bit-identical results
+            assert torch.equal(ys[i], refs[i]), (rep, i)
     plan.close()
     fresh.close()
+
+
+@pytest.mark.parametrize("tbsize", [5, 7])
+def test_forward_is_bit_reproducible(tbsize):
+    # Fixed-order overlap-add at every radius: two runs give the same bits.
+    announce("bit-reproducible forward, tbsize", tbsize)
+    p = dfttest_params(tbsize=tbsize)
+    x = synth.float_plane(6, 70, 90, seed=31).cuda()
+    plan = _lib.Plan(p, _lib.DFT2_SAMPLE_FLOAT, 32)
+    outs = []
+    for _ in range(2):
+        y = torch.empty_like(x)
+        plan.process_batch([plane_job(x, y)])
+        torch.cuda.synchronize()
+        outs.append(y)
+    plan.close()
+    assert torch.equal(outs[0], outs[1])
+    assert max_err(outs[0].cpu().numpy(), onp.process_plane(x.cpu().numpy(), p, 32, True)) <= 1e-5
 
 
 @pytest.mark.parametrize("fmt,precision,asym", [("u8", "fp32", False), ("f32", "fp32", True),
