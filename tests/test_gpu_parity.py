@@ -111,6 +111,27 @@ def test_full_scale_8bit_content(pattern, precision):
     assert max_err(out, onp.process_plane(x, p, 8, False)) <= 1
 
 
+@pytest.mark.parametrize("precision", ["fp32", "split_bf16"])
+@pytest.mark.parametrize("bits,pattern", [(16, "checker"), (16, "binary_noise"), (10, "binary_noise")])
+def test_full_scale_16bit_content(bits, pattern, precision):
+    # 9- to 16-bit samples enter the first contraction as their two bytes, each exact in
+    # BF16 (both precisions); full-scale content exercises both bytes at their extremes.
+    announce("full-scale u16", bits, pattern, precision)
+    n, h, w = 5, 52, 68
+    peak = (1 << bits) - 1
+    rng = np.random.default_rng(13)
+    if pattern == "checker":
+        yy, xx = np.indices((h, w))
+        x = np.stack([((yy + xx + t) % 2) * peak for t in range(n)])
+    else:
+        x = rng.integers(0, 2, (n, h, w)) * peak
+    x = x.astype(np.uint16)
+    p = dfttest_params(tbsize=5)
+    out = gpu_filter(x, p, bits, False, Backend.MI355X(precision=precision))
+    bound = 1 if precision == "fp32" or bits <= 12 else 4
+    assert max_err(out, onp.process_plane(x, p, bits, False)) <= bound
+
+
 def test_asymmetric_sigma_array():
     announce("asymmetric sigma")
     p = dfttest_params()
