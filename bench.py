@@ -104,10 +104,12 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "split_bf16"], default="fp32")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
-    ap.add_argument("--workload", choices=["fp32_1080p", "precision4k", "batch", "diff"], default="fp32_1080p",
+    ap.add_argument("--workload", choices=["fp32_1080p", "precision4k", "batch", "diff", "frame_host"],
+                    default="fp32_1080p",
                     help="fp32_1080p: the headline line; precision4k: 4K 10-bit FP32 vs split-BF16 with "
                          "errors vs FP64; batch: 64 uneven float clips as one autotuned graph launch; "
-                         "diff: forward vs forward+backward of the differentiable filter")
+                         "diff: forward vs forward+backward of the differentiable filter; frame_host: "
+                         "per-frame calls on host planes (PCIe included)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the path has no multi-GPU form here (use --gpus 1)",
@@ -132,6 +134,8 @@ def main():
         return bench_diff(args)
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "frame_host":
+        return bench_frame_host(args)
 
     # Whole clip in one fused launch: scratch for 240 frames of tile regions.
     os.environ.setdefault("DFT2_SCRATCH_MB", "8192")
@@ -357,6 +361,64 @@ def bench_batch(args):
                                     for c in cands]},
     }), flush=True)
     batch.close()
+
+
+def bench_frame_host(args):
+    """The reference's own call pattern: one dft2_process_frame call per output frame and
+    plane on host planes (the T source frames copied in, the output copied back), as
+    DFTTestGetFrame runs it (hiprtc_source/source.cpp:445-692).  1080p YUV420 u8, 48 frames;
+    one caller thread on one stream slot, and four threads on four slots (VapourSynth's
+    fmParallel); pageable (numpy) and pinned (hipHostMalloc through torch) host memory.
+    PCIe-inclusive frames per second; never the headline value."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    from dfttest2_amd import _lib, synth
+    from dfttest2_amd.params import dfttest_params
+
+    frames = 48
+    p = dfttest_params()
+    t_len, r = p.t_len, p.t_len // 2
+    base = synth.yuv420_planes(frames, WIDTH, HEIGHT, seed=1, bits=8, sd_lsb=6.0)
+    results = {}
+    for memory in ("pageable", "pinned"):
+        if memory == "pinned":
+            planes = [b.pin_memory() for b in base]
+            outs = [torch.empty_like(b).pin_memory() for b in base]
+            ptr = lambda a, n: a[n].data_ptr()  # noqa: E731
+        else:
+            planes = [np.ascontiguousarray(b.numpy()) for b in base]
+            outs = [np.empty_like(a) for a in planes]
+            ptr = lambda a, n: a[n].ctypes.data  # noqa: E731
+
+        def run(plan, frame_ids):
+            for n in frame_ids:
+                for pl, out in zip(planes, outs):
+                    _, h, w = pl.shape
+                    srcs = [ptr(pl, min(max(n - r + t, 0), frames - 1)) for t in range(t_len)]
+                    plan.process_frame(srcs, w, ptr(out, n), w, w, h, memory=1)
+
+        for threads in (1, 4):
+            plan = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8, 0, threads, 0)
+            run(plan, range(4))
+            t0 = time.perf_counter()
+            if threads == 1:
+                run(plan, range(frames))
+            else:
+                pool = [threading.Thread(target=run, args=(plan, range(i, frames, threads))) for i in range(threads)]
+                for th in pool:
+                    th.start()
+                for th in pool:
+                    th.join()
+            dt = time.perf_counter() - t0
+            plan.close()
+            results[f"{memory}_{threads}thread"] = {"fps": round(frames / dt, 1), "ms_per_frame": round(dt / frames * 1e3, 3)}
+            print(f"[bench] frame_host {memory}, {threads} thread(s): {frames / dt:.1f} fps", file=sys.stderr, flush=True)
+    print(json.dumps({"metric": METRIC, "workload": "frame_host: dft2_process_frame per frame and plane on host "
+                      "planes, 1080p YUV420 u8, 48 frames, DFTTest defaults", "unit": "fps",
+                      "results": results}), flush=True)
 
 
 def bench_diff(args):

@@ -311,6 +311,17 @@ int ensure_host(void** ptr, size_t* cap, size_t bytes) {
     return DFT2_OK;
 }
 
+// True for memory the GPU reaches directly (pinned host, device or managed); false for
+// ordinary pageable host memory, whose attributes query fails or reports it unregistered.
+bool device_visible(const void* ptr) {
+    hipPointerAttribute_t at{};
+    if (hipPointerGetAttributes(&at, ptr) != hipSuccess) {
+        (void)hipGetLastError(); // the query's error is not the caller's
+        return false;
+    }
+    return at.type != hipMemoryTypeUnregistered;
+}
+
 int check_mirror(int size, int step, const char* what) {
     int ps = pad_size(size, 16, step);
     int o = (ps - size) / 2;
@@ -797,6 +808,9 @@ int dft2_plan_destroy(dft2_plan* p) {
         if (s.stage) {
             (void)hipFree(s.stage);
         }
+        if (s.h_stage) {
+            (void)hipHostFree(s.h_stage);
+        }
         if (s.stream) {
             (void)hipStreamDestroy(s.stream);
         }
@@ -1022,8 +1036,17 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     }
     // Host memory: stage T planes and the output through device memory (the reference
     // copies T padded planes host->device per frame, hiprtc_source/source.cpp:570-573).
+    // Pinned planes go over with 2-D copies.  Pageable planes (VapourSynth's frames) are
+    // first copied row by row into the slot's pinned buffer and moved with one copy each
+    // way, as the reference stages through its per-thread pinned h_padded
+    // (hiprtc_source/source.cpp:480-510): 2-D copies from pageable memory run at a small
+    // fraction of the link rate (74 against 0.7 ms per 1080p YUV420 frame).
     const long long row = static_cast<long long>(width) * bps;
     const long long plane = row * height;
+    bool pinned = device_visible(dst);
+    for (int t = 0; t < T && pinned; t++) {
+        pinned = device_visible(src_frames[t]);
+    }
     Slot* slot = acquire_slot(p);
     SlotGuard guard{p, slot};
     hipStream_t st = slot->stream;
@@ -1033,9 +1056,25 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         return rc;
     }
     char* stage = static_cast<char*>(slot->stage);
-    for (int t = 0; t < T; t++) {
-        HIP_TRY(hipMemcpy2DAsync(stage + t * plane, row, src_frames[t], src_stride, row, height,
-                                 hipMemcpyHostToDevice, st));
+    char* h_stage = nullptr;
+    if (pinned) {
+        for (int t = 0; t < T; t++) {
+            HIP_TRY(hipMemcpy2DAsync(stage + t * plane, row, src_frames[t], src_stride, row, height,
+                                     hipMemcpyHostToDevice, st));
+        }
+    } else {
+        rc = ensure_host(&slot->h_stage, &slot->h_stage_bytes, static_cast<size_t>(plane) * (T + 1));
+        if (rc) {
+            return rc;
+        }
+        h_stage = static_cast<char*>(slot->h_stage);
+        for (int t = 0; t < T; t++) {
+            const char* src = static_cast<const char*>(src_frames[t]);
+            for (int y = 0; y < height; y++) {
+                std::memcpy(h_stage + t * plane + y * row, src + y * src_stride, static_cast<size_t>(row));
+            }
+        }
+        HIP_TRY(hipMemcpyAsync(stage, h_stage, static_cast<size_t>(plane) * T, hipMemcpyHostToDevice, st));
     }
     uj.src = stage;
     uj.src_stride = row;
@@ -1051,9 +1090,20 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     if (rc) {
         return rc;
     }
-    HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, stage + T * plane, row, row, height, hipMemcpyDeviceToHost, st));
+    if (pinned) {
+        HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, stage + T * plane, row, row, height, hipMemcpyDeviceToHost, st));
+    } else {
+        HIP_TRY(hipMemcpyAsync(h_stage + T * plane, stage + T * plane, static_cast<size_t>(plane),
+                               hipMemcpyDeviceToHost, st));
+    }
     HIP_TRY(hipEventRecord(slot->done, st));
     HIP_TRY(hipStreamSynchronize(st));
+    if (!pinned) {
+        char* out = static_cast<char*>(dst);
+        for (int y = 0; y < height; y++) {
+            std::memcpy(out + y * dst_stride, h_stage + T * plane + y * row, static_cast<size_t>(row));
+        }
+    }
     return DFT2_OK;
 }
 

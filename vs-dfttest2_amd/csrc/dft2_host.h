@@ -60,6 +60,8 @@ struct Slot {
     size_t tab_dev_cap = 0;
     void* stage = nullptr; // host-memory path: T input planes + output plane
     size_t stage_bytes = 0;
+    void* h_stage = nullptr; // host-memory path, pageable planes: pinned copy of the same
+    size_t h_stage_bytes = 0;
     bool busy = false;
 };
 
