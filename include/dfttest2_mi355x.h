@@ -136,7 +136,9 @@ DFT2_API int dft2_plan_destroy(dft2_plan* plan);
  * planes in temporal order (already clamped by the caller, as GetFrame gathers them).
  * memory = 0: device pointers, work is queued on `stream` (hipStream_t; NULL = the
  * plan's own stream, and the call waits for completion).  memory = 1: host pointers;
- * the call stages through device memory and returns when dst is written. */
+ * the call stages through device memory and returns when dst is written.  Pageable host
+ * planes pass through a pinned buffer of the call's stream slot; pinned ones are copied
+ * directly. */
 DFT2_API int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, long long src_stride, void* dst,
                        long long dst_stride, int width, int height, int memory, void* stream);
 
