@@ -171,6 +171,29 @@ def test_frame_entry_host_and_device_match_batch():
         np.testing.assert_array_equal(dout.cpu().numpy(), full[fr])
 
 
+def test_frame_entry_pinned_host_planes():
+    # Pinned host planes take the direct copies, pageable ones the pinned staging buffer:
+    # both give the batch result, also with row strides wider than the plane.
+    announce("process_frame pinned host planes")
+    p = dfttest_params()
+    x, bits, _ = make_plane("u8", (5, 46, 58), 43)
+    full = gpu_filter(x, p, bits, False)
+    plan = get_plan(p, "int", 8, Backend.MI355X())
+    n, h, w = x.shape
+    pitch = 64
+    wide = torch.zeros((n, h, pitch), dtype=torch.uint8).pin_memory()
+    wide[:, :, :w] = torch.from_numpy(x)
+    for fr in (0, 2, 4):
+        idx = [min(max(fr - 1 + t, 0), n - 1) for t in range(3)]
+        out = torch.zeros((h, pitch), dtype=torch.uint8).pin_memory()
+        plan.process_frame([wide[i].data_ptr() for i in idx], pitch, out.data_ptr(), pitch, w, h, memory=1)
+        np.testing.assert_array_equal(out[:, :w].numpy(), full[fr])
+        paged = np.zeros((h, pitch), np.uint8)
+        srcs = [np.ascontiguousarray(wide[i].numpy()) for i in idx]
+        plan.process_frame([s_.ctypes.data for s_ in srcs], pitch, paged.ctypes.data, pitch, w, h, memory=1)
+        np.testing.assert_array_equal(paged[:, :w], full[fr])
+
+
 def test_batch_of_uneven_clips_matches_single_runs():
     announce("batch uneven clips")
     p = dfttest_params()
