@@ -273,6 +273,22 @@ def test_full_hd_identity_is_exact(fmt):
     assert torch.equal(out.to(torch.int32), x.to(torch.int32))
 
 
+@pytest.mark.parametrize("bits", [8, 16])
+def test_8k_identity_is_exact(bits, monkeypatch):
+    # The identity at 7680x4320 with the scratch limited to 256 MiB: many tiles per frame,
+    # launches split into chunks, and (16-bit) the byte-split x transform at full scale.
+    announce("8K identity", bits)
+    monkeypatch.setenv("DFT2_SCRATCH_MB", "256")
+    p = dfttest_params(ftype=2, sigma=1.0)
+    x = synth.int_plane(3, 4320, 7680, 5, bits, 300.0 if bits == 16 else 6.0, device="cuda")
+    out = torch.empty_like(x)
+    plan = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, bits)
+    plan.process_batch([plane_job(x, out)], torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    plan.close()
+    assert torch.equal(out.to(torch.int32), x.to(torch.int32))
+
+
 def test_full_hd_constant_frames_pass_through():
     announce("1080p constant")
     p = dfttest_params(ftype=1, tbsize=5)
