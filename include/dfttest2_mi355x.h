@@ -135,7 +135,10 @@ DFT2_API int dft2_plan_destroy(dft2_plan* plan);
 /* Filters one output frame of one plane.  src_frames holds the T = 2*radius+1 source
  * planes in temporal order (already clamped by the caller, as GetFrame gathers them).
  * memory = 0: device pointers, work is queued on `stream` (hipStream_t; NULL = the
- * plan's own stream, and the call waits for completion).  memory = 1: host pointers;
+ * plan's own stream, and the call waits for completion; that stream does not wait for
+ * other streams, so the planes must be complete when the call is made).  Calls from
+ * several threads on one plan are safe; at most num_streams run at once, the others
+ * wait for a free stream slot.  memory = 1: host pointers;
  * the call stages through device memory and returns when dst is written.  Pageable host
  * planes pass through a pinned buffer of the call's stream slot; pinned ones are copied
  * directly. */

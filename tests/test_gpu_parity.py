@@ -167,6 +167,7 @@ def test_frame_entry_host_and_device_match_batch():
         np.testing.assert_array_equal(out, full[fr])
         dev = [torch.from_numpy(s).cuda() for s in srcs]
         dout = torch.zeros((h, w), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()  # the plan's own stream does not wait for torch's
         plan.process_frame([d.data_ptr() for d in dev], w, dout.data_ptr(), w, w, h, memory=0)
         np.testing.assert_array_equal(dout.cpu().numpy(), full[fr])
 
@@ -192,6 +193,50 @@ def test_frame_entry_pinned_host_planes():
         srcs = [np.ascontiguousarray(wide[i].numpy()) for i in idx]
         plan.process_frame([s_.ctypes.data for s_ in srcs], pitch, paged.ctypes.data, pitch, w, h, memory=1)
         np.testing.assert_array_equal(paged[:, :w], full[fr])
+
+
+def test_frame_entry_concurrent_threads():
+    # fmParallel: VapourSynth calls GetFrame for different frames from several threads at once
+    # (hiprtc_source/source.cpp:445-472, tickets over num_streams at :559-566).  Six threads
+    # share one plan with two stream slots, each filtering its own frames from pageable host
+    # planes and from device planes; every output equals the clip result.
+    import threading
+
+    announce("process_frame from 6 threads, num_streams 2")
+    p = dfttest_params()
+    x, bits, _ = make_plane("u8", (12, 70, 90), 47)
+    full = gpu_filter(x, p, bits, False)
+    plan = get_plan(p, "int", 8, Backend.MI355X(num_streams=2))
+    n, h, w = x.shape
+    dev_x = torch.from_numpy(x).cuda()
+    torch.cuda.synchronize()
+    errors = []
+
+    def worker(k):
+        try:
+            torch.cuda.set_device(0)
+            for rep in range(3):
+                for fr in range(k, n, 6):
+                    idx = [min(max(fr - 1 + t, 0), n - 1) for t in range(3)]
+                    out = np.zeros((h, w), np.uint8)
+                    srcs = [np.ascontiguousarray(x[i]) for i in idx]
+                    plan.process_frame([s.ctypes.data for s in srcs], w, out.ctypes.data, w, w, h, memory=1)
+                    if not np.array_equal(out, full[fr]):
+                        errors.append(f"host frame {fr} thread {k} rep {rep}")
+                    dout = torch.empty((h, w), dtype=torch.uint8, device="cuda")  # no fill kernel to wait for
+                    plan.process_frame([dev_x[i].data_ptr() for i in idx], w, dout.data_ptr(), w, w, h, memory=0)
+                    if not np.array_equal(dout.cpu().numpy(), full[fr]):
+                        errors.append(f"device frame {fr} thread {k} rep {rep}")
+        except Exception as e:  # reported below, in the main thread
+            errors.append(f"thread {k}: {e!r}")
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(6)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=90)
+    assert not any(th.is_alive() for th in threads)
+    assert not errors, errors[:5]
 
 
 def test_batch_of_uneven_clips_matches_single_runs():
