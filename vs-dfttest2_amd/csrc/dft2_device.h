@@ -397,9 +397,10 @@ __device__ __forceinline__ gptr_t<Tin> to_global(const Tin* p) {
 template <int FTM>
 __device__ __forceinline__ float gain_of(float psd, float s, const LaunchArgs& a) {
     if constexpr (FTM == 0) {
-        // s arrives as sigma + 1e-15 (gain table of a Wiener plan):
+        // s arrives as sigma + 1e-15 (gain table of a Wiener plan) and psd as
+        // |x|^2 + 1e-15 (filter_coef):
         // (psd - sigma) / (psd + 1e-15) = 1 - (sigma + 1e-15) / (psd + 1e-15).
-        return fmaxf(fmaf(-s, __builtin_amdgcn_rcpf(psd + 1e-15f), 1.0f), 0.0f);
+        return fmaxf(fmaf(-s, __builtin_amdgcn_rcpf(psd), 1.0f), 0.0f);
     } else if constexpr (FTM == 1) {
         return psd < s ? 0.0f : 1.0f;
     } else {
@@ -446,7 +447,8 @@ template <int FTM, int ASYM>
 __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, float gf, const LaunchArgs& a) {
     float xr = re - gf * e.z;
     float xi = im - gf * e.w;
-    float psd = xr * xr + xi * xi;
+    // The Wiener gain takes psd + 1e-15: the constant rides in the psd's own fma chain.
+    float psd = fmaf(xr, xr, fmaf(xi, xi, FTM == 0 ? 1e-15f : 0.0f));
     float g = gain_of<FTM>(psd, e.x, a);
     if constexpr (ASYM != 0) {
         if ((ASYM == 1 || a.asym) && e.y != e.x) { // only the kx = 0, 8 lanes
