@@ -423,11 +423,6 @@ __device__ __forceinline__ float gain_of(float psd, float s, const LaunchArgs& a
     }
 }
 
-// Zero-mean, gain and mean re-add on one complex coefficient.  e = (sigma, sigma of the
-// conjugate partner, Re WF, Im WF) at this coefficient.  With an asymmetric sigma array
-// the coefficients at kx = 0 and kx = 8 take the mean of their own gain and their
-// conjugate partner's gain, which is what the reference's C2R (it drops Im at those
-// columns) makes of two different gains.
 // One gain-table entry from LDS as a single 128-bit read.  Without the empty asm the
 // symmetric form, which does not use the partner sigma, gets split 32/64-bit reads at a
 // 16-byte lane stride: twice the LDS instructions, with bank conflicts.
@@ -440,23 +435,51 @@ __device__ __forceinline__ float4 tab_entry(const float4* p) {
     return e;
 }
 
+// 1 - gain of the Wiener (FTM 0) and hard-threshold (FTM 1) filters, with psd and s as
+// gain_of takes them: Wiener 1 - max(1 - s/psd, 0) = min(s/psd, 1).
+template <int FTM>
+__device__ __forceinline__ float damping_of(float psd, float s) {
+    if constexpr (FTM == 0) {
+        return fminf(s * __builtin_amdgcn_rcpf(psd), 1.0f);
+    } else {
+        return psd < s ? 1.0f : 0.0f;
+    }
+}
+
+// Zero-mean, gain and mean re-add on one complex coefficient.  e = (sigma, sigma of the
+// conjugate partner, Re WF, Im WF) at this coefficient.  With an asymmetric sigma array
+// the coefficients at kx = 0 and kx = 8 take the mean of their own gain and their
+// conjugate partner's gain, which is what the reference's C2R (it drops Im at those
+// columns) makes of two different gains.
 // ASYM: 0 = symmetric sigma (no partner gains), 1 = asymmetric, 2 = decided at run time
 // from a.asym.  The forms are separate kernel instantiations because the compiler turns
 // a run-time test into selects and computes the partner gains for every plan.
 template <int FTM, int ASYM>
 __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, float gf, const LaunchArgs& a) {
-    float xr = re - gf * e.z;
-    float xi = im - gf * e.w;
+    const float xr = fmaf(-gf, e.z, re);
+    const float xi = fmaf(-gf, e.w, im);
     // The Wiener gain takes psd + 1e-15: the constant rides in the psd's own fma chain.
-    float psd = fmaf(xr, xr, fmaf(xi, xi, FTM == 0 ? 1e-15f : 0.0f));
-    float g = gain_of<FTM>(psd, e.x, a);
-    if constexpr (ASYM != 0) {
-        if ((ASYM == 1 || a.asym) && e.y != e.x) { // only the kx = 0, 8 lanes
-            g = 0.5f * (g + gain_of<FTM>(psd, e.y, a));
+    const float psd = fmaf(xr, xr, fmaf(xi, xi, FTM == 0 ? 1e-15f : 0.0f));
+    if constexpr (FTM == 0 || FTM == 1) {
+        // g x + gf WF = (x + gf WF) - (1 - g) x: the mean term gf WF is never formed.
+        float h = damping_of<FTM>(psd, e.x);
+        if constexpr (ASYM != 0) {
+            if ((ASYM == 1 || a.asym) && e.y != e.x) { // only the kx = 0, 8 lanes
+                h = 0.5f * (h + damping_of<FTM>(psd, e.y));
+            }
         }
+        re = fmaf(-xr, h, re);
+        im = fmaf(-xi, h, im);
+    } else {
+        float g = gain_of<FTM>(psd, e.x, a);
+        if constexpr (ASYM != 0) {
+            if ((ASYM == 1 || a.asym) && e.y != e.x) {
+                g = 0.5f * (g + gain_of<FTM>(psd, e.y, a));
+            }
+        }
+        re = fmaf(xr, g, gf * e.z);
+        im = fmaf(xi, g, gf * e.w);
     }
-    re = xr * g + gf * e.z;
-    im = xi * g + gf * e.w;
 }
 
 // Coefficient (kt, ky, kx) owned by middle-stage lane `lane` at temporal pair g and slot
