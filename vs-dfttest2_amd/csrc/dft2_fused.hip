@@ -116,6 +116,14 @@ void fused_kernel(LaunchArgs a) {
     // SEP: sample scale * temporal window factor of the centre slice; the other slices'
     // factors are folded into the temporal basis (WTS).
     const float as_c = SEP ? a.consts[kOffAS + R] : 1.0f;
+    // Centre-slice weights of the temporal inverse: c0 for the DC pair and 2 c0 for the
+    // others (c0 = 1, or the centre slice's temporal synthesis factor when separable).
+    // c0 rides in the inverse x operand, the 2 in the middle stage's fma.
+    const float c0 = a.consts[SEP ? kOffCTS : kOffCT];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        gx[s] *= c0;
+    }
     const Opnd WXo = make_opnd<PREC>(wx[0], wx[1], wx[2], wx[3]);
     const Opnd3 WX3 = make_opnd3(wx[0], wx[1], wx[2], wx[3]); // FP32 with 8-bit samples
     const OpndW WXw = make_opndw<PREC>(wx[0], wx[1], wx[2], wx[3]); // 16-bit samples
@@ -123,10 +131,8 @@ void fused_kernel(LaunchArgs a) {
     const Opnd GXo = make_opnd<PREC>(gx[0], gx[1], gx[2], gx[3]);
     const Opnd GYo = make_opnd<PREC>(gy[0], gy[1], gy[2], gy[3]);
     float wt[T][T];
-    float ct[T];
 #pragma unroll
     for (int j = 0; j < T; j++) {
-        ct[j] = a.consts[(SEP ? kOffCTS : kOffCT) + j];
 #pragma unroll
         for (int t = 0; t < T; t++) {
             wt[j][t] = a.consts[(SEP ? kOffWTS : kOffWT) + j * T + t];
@@ -394,22 +400,21 @@ void fused_kernel(LaunchArgs a) {
             }
 
             // Back to the real (t, y) basis at the centre slice, where the sin-t rows
-            // vanish: only the cos-t components, times the centre weights.
-            const float wc_ = ct[g];
+            // vanish: only the cos-t components, times the centre weights (1 for g = 0,
+            // 2 otherwise; their common factor c0 is in the inverse x operand).
+            const float wc_ = g ? 2.0f : 1.0f;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 const float pr = zr[0][c], pi = zi[0][c], qr = zr[1][c], qi = zi[1][c];
                 // doubled for a proper y pair (see gy above)
                 const float u_c0 = fmaf(eyf, qr, pr); // cos t, y slot 0 (pr + qr for a proper pair)
                 const float u_c1 = ey ? qi - pi : qr; // cos t, y slot 1
-                const float y0 = wc_ * u_c0;
-                const float y1 = wc_ * u_c1;
                 if (c == 0) {
-                    v00 += y0;
-                    v10 += y1;
+                    v00 = fmaf(wc_, u_c0, v00);
+                    v10 = fmaf(wc_, u_c1, v10);
                 } else {
-                    v01 += y0;
-                    v11 += y1;
+                    v01 = fmaf(wc_, u_c0, v01);
+                    v11 = fmaf(wc_, u_c1, v11);
                 }
             }
         }
