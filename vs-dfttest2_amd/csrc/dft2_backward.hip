@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
         __syncthreads();
         {
             const int Sg = opaque_uniform(S); // per-lane bounds stay in the loop
-            const int Sm = (65536 + Sg - 1) / Sg; // for div_small
+            const int Sm = opaque_uniform(a.step_magic); // for div_small
             const int nb = min(kWaves, nbx - pass * kWaves);
             const int cols = Sg * (nb - 1) + 16;
             for (int x0 = 0; x0 < cols; x0 += 64) {

@@ -336,6 +336,7 @@ LaunchArgs make_args(const dft2_plan* p, const TileShape& ts) {
     LaunchArgs a{};
     a.consts = p->d_consts;
     a.block_step = p->block_step;
+    a.step_magic = (65536 + p->block_step - 1) / p->block_step;
     a.tile_bw = ts.bw;
     a.tile_bh = ts.bh;
     a.region_w = ts.region_w;

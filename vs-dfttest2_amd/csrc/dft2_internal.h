@@ -81,6 +81,7 @@ struct LaunchArgs {
     int n_jobs;
     float* scratch;             // per work item: region_h * region_w floats
     int block_step;
+    int step_magic;             // ceil(65536 / block_step): x / block_step by div_small
     int tile_bw, tile_bh;       // tile size in blocks
     int region_w, region_h;     // accumulator region of a full tile
     float inv_wf0;              // 1 / Re window_freq[0][0][0]
