@@ -510,13 +510,28 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
                 // bounds and masks serve the wave's rows y = wave, wave + 8 of every slice
                 const int lo = div_small(max(x + Sg - 16, 0), Sm);
                 const int hi = min(div_small(x, Sm), nb - 1);
+                // All reads first (one LDS round trip), then the sums and the writes.
+                constexpr int kRows = 16 / kWaves;
+                const int k0 = max(hi - 3, 0);
+                float old[kRows][T], sum[kRows][T];
 #pragma unroll
-                for (int yy = 0; yy < 16; yy += kWaves) {
-                    const int y = yy + wave;
+                for (int i = 0; i < kRows; i++) {
+                    const int y = i * kWaves + wave;
                     const int g = (r * Sg + y) * rw + pass * kWaves * Sg + x;
 #pragma unroll
                     for (int t = 0; t < T; t++) {
-                        gacc[t * region + g] += gather_sum<kWbuf>(stage + t * 256 + y * 16 + x, lo, hi);
+                        old[i][t] = gacc[t * region + g];
+                        sum[i][t] = gather_head<kWbuf>(stage + t * 256 + y * 16 + x, k0, lo, hi);
+                    }
+                }
+#pragma unroll
+                for (int i = 0; i < kRows; i++) {
+                    const int y = i * kWaves + wave;
+                    const int g = (r * Sg + y) * rw + pass * kWaves * Sg + x;
+#pragma unroll
+                    for (int t = 0; t < T; t++) {
+                        sum[i][t] = gather_tail<kWbuf>(stage + t * 256 + y * 16 + x, k0, lo, sum[i][t]);
+                        gacc[t * region + g] = old[i][t] + sum[i][t];
                     }
                 }
             }

@@ -229,9 +229,11 @@ __device__ __forceinline__ int div_small(int x, int m) {
 // read together at immediate offsets (for block steps >= 4 they include every covering
 // block) and masked to lo..hi; block steps 1 and 2 add the rest one by one.  Every read
 // stays below block max(hi, 3) <= 7, inside the staging area.
+// The gather is split in two, gather_head (blocks k0..k0+3, k0 = max(hi - 3, 0)) and
+// gather_tail (blocks k0-1 down to lo), so that a caller with several pixels issues every
+// first-part read, and its accumulator reads, before any sum is formed.
 template <int kStride>
-__device__ __forceinline__ float gather_sum(const float* p, int lo, int hi) {
-    const int k0 = max(hi - 3, 0);
+__device__ __forceinline__ float gather_head(const float* p, int k0, int lo, int hi) {
     const float* q = p + __umul24(static_cast<unsigned>(k0), static_cast<unsigned>(kStride));
     float sum = 0.0f;
 #pragma unroll
@@ -240,11 +242,17 @@ __device__ __forceinline__ float gather_sum(const float* p, int lo, int hi) {
         // lo <= k0 + j <= hi as one compare, so that the read is not made conditional
         sum += static_cast<unsigned>(k0 + j - lo) <= static_cast<unsigned>(hi - lo) ? v : 0.0f;
     }
+    return sum;
+}
+
+template <int kStride>
+__device__ __forceinline__ float gather_tail(const float* p, int k0, int lo, float sum) {
     for (int k = k0 - 1; k >= lo; k--) {
         sum += p[k * kStride];
     }
     return sum;
 }
+
 
 __device__ __forceinline__ int mirror(int v, int n) {
     v = v < 0 ? -v : v;

@@ -463,9 +463,20 @@ void fused_kernel(LaunchArgs a) {
                 const int lo = div_small(max(x + Sg - 16, 0), Sm);
                 const int hi = min(div_small(x, Sm), nb - 1);
                 float* acol = acc + (r * Sg + wave) * rw + pass * kWaves * Sg + x;
+                // All reads of the wave's rows (accumulator and staged values) first, one
+                // LDS round trip, then the sums and the writes.
+                constexpr int kRows = 16 / kWaves;
+                const int k0 = max(hi - 3, 0);
+                float old[kRows], sum[kRows];
 #pragma unroll
-                for (int yy = 0; yy < 16; yy += kWaves) {
-                    acol[yy * rw] += gather_sum<kWbuf>(stage + (yy + wave) * kOut + x, lo, hi);
+                for (int i = 0; i < kRows; i++) {
+                    old[i] = acol[i * kWaves * rw];
+                    sum[i] = gather_head<kWbuf>(stage + (i * kWaves + wave) * kOut + x, k0, lo, hi);
+                }
+#pragma unroll
+                for (int i = 0; i < kRows; i++) {
+                    sum[i] = gather_tail<kWbuf>(stage + (i * kWaves + wave) * kOut + x, k0, lo, sum[i]);
+                    acol[i * kWaves * rw] = old[i] + sum[i];
                 }
             }
         }
