@@ -218,7 +218,9 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
                     }
                 }
             }
-            if (sep) {
+            // The separable form scales the slices by the centre slice's factor and the
+            // temporal basis by the others' ratios to it.
+            if (sep && at[r] != 0.0) {
                 p->separable = 1;
                 for (int x = 0; x < 16; x++) {
                     for (int n = 0; n < 16; n++) {
@@ -234,7 +236,7 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
                     c[kOffCTS + j] = static_cast<float>((j == 0 ? 1.0 : (j <= r ? 2.0 : 0.0)) * at[r]);
                     for (int k = 1; k <= r; k++) {
                         const double v = j <= r ? std::cos(2 * pi * j * k / T) : std::sin(2 * pi * (j - r) * k / T);
-                        c[kOffWTS + j * T + k] = static_cast<float>(v * static_cast<double>(scale) * at[r + k]);
+                        c[kOffWTS + j * T + k] = static_cast<float>(v * at[r + k] / at[r]);
                     }
                 }
             }

@@ -113,9 +113,16 @@ void fused_kernel(LaunchArgs a) {
             }
         }
     }
-    // SEP: sample scale * temporal window factor of the centre slice; the other slices'
-    // factors are folded into the temporal basis (WTS).
-    const float as_c = SEP ? a.consts[kOffAS + R] : 1.0f;
+    // SEP: sample scale * temporal window factor of the centre slice, applied to every
+    // slice through the x analysis operand; the other slices' factors relative to it are
+    // in the temporal basis (WTS).
+    if constexpr (SEP) {
+        const float as_c = a.consts[kOffAS + R];
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            wx[s] *= as_c;
+        }
+    }
     // Centre-slice weights of the temporal inverse: c0 for the DC pair and 2 c0 for the
     // others (c0 = 1, or the centre slice's temporal synthesis factor when separable).
     // c0 rides in the inverse x operand, the 2 in the middle stage's fma.
@@ -275,7 +282,7 @@ void fused_kernel(LaunchArgs a) {
         }
 
         // ---- forward: D2_t = WY * (slice_t * WX) ----
-        // Separable window: the slices enter unscaled (their factors are in WTS and as_c).
+        // Separable window: the slices enter unscaled (their factors are in WX and WTS).
         // 8-bit samples are exact in BF16, so the first contraction takes them as they are:
         // split BF16, two MFMA and no split; FP32, the BF16 MFMA over the exact three-part
         // split of WX (three MFMA, every product exact, FP32 accumulation).  16-bit samples
@@ -313,7 +320,7 @@ void fused_kernel(LaunchArgs a) {
             tsum[k] = d2[R + k] + d2[R - k];
             tdif[k] = d2[R + k] - d2[R - k];
         }
-        const floatx4 dc = SEP ? d2[R] * as_c : d2[R];
+        const floatx4 dc = d2[R];
 #pragma unroll
         for (int j = 0; j < T; j++) {
             floatx4 u;

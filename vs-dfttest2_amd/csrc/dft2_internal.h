@@ -46,7 +46,9 @@ constexpr int kTabFloats(int radius) { return (radius + 1) * 4 * 64 * 4; }
 //   AS  [t]    = sample scale * a[t]   (per-slice factor of the analysis window; a[t] is
 //                symmetric about the centre, a[r + k] = a[r - k], for the separable form)
 //   CTS [j]    = CT[j] * a[r]          (centre weights with the synthesis window's a[r])
-//   WTS [j][k] = WT[j][k] * AS[r + k]  (temporal basis with the slice factors folded in)
+//   WTS [j][k] = WT[j][k] * a[r + k] / a[r]  (temporal basis with the slice factors
+//                relative to the centre slice's; the kernel scales its x analysis operand
+//                by AS[r], which every slice then carries; a[r] != 0 for the form)
 constexpr int kOffSWX = kOffTAB + 4 * 4 * 64 * 4;
 constexpr int kOffSWY = kOffSWX + 256;
 constexpr int kOffAS = kOffSWY + 256;         // up to 7, padded to 8
