@@ -28,10 +28,10 @@ WIDTH, HEIGHT, FRAMES = 1920, 1080, 240
 PEAK_FP32_MFMA_TFLOPS = 157.3
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # dense
 # HBM bytes per fused launch on the headline workload (one launch per step): rocprofv3
-# --pmc FETCH_SIZE 4,567,110 KiB x 2 + WRITE_SIZE 4,797,210 KiB x 1, the factors being what
+# --pmc FETCH_SIZE 4,562,310 KiB x 2 + WRITE_SIZE 4,797,210 KiB x 1, the factors being what
 # the same counters read for a plain copy kernel of known size
 # (profiles/fused_kernel_1080p_yuv420_u8_fp32.md).  Valid for the FP32 default workload.
-FUSED_TRAFFIC_BYTES = (4_567_110 * 2 + 4_797_210) * 1024
+FUSED_TRAFFIC_BYTES = (4_562_310 * 2 + 4_797_210) * 1024
 
 
 def flop_per_block(t_len: int) -> int:
