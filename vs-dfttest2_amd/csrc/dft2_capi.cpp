@@ -256,6 +256,13 @@ void build_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c, f
             }
         }
     }
+    // The symmetric Wiener form clamps its damping factor sigma / psd to [0, 1], which holds
+    // for sigma >= 0 only; a negative sigma (gains above 1) takes the other form.
+    for (int i = 0; i < prm->sigma_len && p->filter_type == 0; i++) {
+        if (prm->sigma[i] < 0.0) {
+            p->asym = 1;
+        }
+    }
 }
 
 Slot* acquire_slot(dft2_plan* p) {

@@ -444,11 +444,14 @@ __device__ __forceinline__ float4 tab_entry(const float4* p) {
 }
 
 // 1 - gain of the Wiener (FTM 0) and hard-threshold (FTM 1) filters, with psd and s as
-// gain_of takes them: Wiener 1 - max(1 - s/psd, 0) = min(s/psd, 1).
-template <int FTM>
+// gain_of takes them: Wiener 1 - max(1 - s/psd, 0) = min(s/psd, 1).  kNonNeg: s >= 0 (the
+// host sends plans with a negative sigma to the asymmetric form), so that the min is a
+// clamp to [0, 1], which folds into the multiply as its output modifier.
+template <int FTM, bool kNonNeg>
 __device__ __forceinline__ float damping_of(float psd, float s) {
     if constexpr (FTM == 0) {
-        return fminf(s * __builtin_amdgcn_rcpf(psd), 1.0f);
+        const float q = s * __builtin_amdgcn_rcpf(psd);
+        return kNonNeg ? fminf(fmaxf(q, 0.0f), 1.0f) : fminf(q, 1.0f);
     } else {
         return psd < s ? 1.0f : 0.0f;
     }
@@ -470,10 +473,10 @@ __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, floa
     const float psd = fmaf(xr, xr, fmaf(xi, xi, FTM == 0 ? 1e-15f : 0.0f));
     if constexpr (FTM == 0 || FTM == 1) {
         // g x + gf WF = (x + gf WF) - (1 - g) x: the mean term gf WF is never formed.
-        float h = damping_of<FTM>(psd, e.x);
+        float h = damping_of<FTM, ASYM == 0>(psd, e.x);
         if constexpr (ASYM != 0) {
             if ((ASYM == 1 || a.asym) && e.y != e.x) { // only the kx = 0, 8 lanes
-                h = 0.5f * (h + damping_of<FTM>(psd, e.y));
+                h = 0.5f * (h + damping_of<FTM, false>(psd, e.y));
             }
         }
         re = fmaf(-xr, h, re);
