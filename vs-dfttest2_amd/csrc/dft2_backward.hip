@@ -45,8 +45,7 @@ __device__ __forceinline__ void gain_adjoint(float ar, float ai, float& dyr, flo
                                              float2& dsig, float& dgf) {
     const float xr = ar - gf * e.z;
     const float xi = ai - gf * e.w;
-    const float psd = xr * xr + xi * xi;
-    const float r = __builtin_amdgcn_rcpf(psd + 1e-15f);
+    const float r = __builtin_amdgcn_rcpf(fmaf(xr, xr, fmaf(xi, xi, 1e-15f))); // 1 / (psd + eps)
     const float c1 = e.x * r;
     const float c2 = e.y * r;
     const bool m1 = c1 < 1.0f; // g1 > 0
