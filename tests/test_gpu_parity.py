@@ -415,6 +415,24 @@ def test_forward_is_bit_reproducible(tbsize):
     assert max_err(outs[0].cpu().numpy(), onp.process_plane(x.cpu().numpy(), p, 32, True)) <= 1e-5
 
 
+@pytest.mark.parametrize("fmt", ["u8", "f32"])
+def test_separable_window_with_zero_centre_slice(fmt):
+    # a[t] b[y] c[x] with a[r] = 0: the separable form scales the temporal basis by 1 / a[r],
+    # so such a window takes the per-lane form; its output is the oracle's (all zero here: the
+    # synthesis window is the centre slice), with no division by zero.
+    announce("separable window, zero centre slice", fmt)
+    p = dfttest_params(twin=0)
+    w = np.asarray(p.window, dtype=np.float64).reshape(p.t_len, 16, 16).copy()
+    w[p.radius] = 0.0
+    p.window = w.reshape(-1)
+    p.window_freq = np.asarray(rdft([v * 255 for v in p.window], (p.t_len, 16, 16)))
+    x, bits, is_float = make_plane(fmt, (5, 40, 44), 19)
+    out = gpu_filter(x, p, bits, is_float)
+    ref = onp.process_plane(x, p, bits, is_float)
+    assert np.isfinite(out.astype(np.float64)).all()
+    assert max_err(out, ref) <= tol(is_float)
+
+
 @pytest.mark.parametrize("fmt,precision,asym", [("u8", "fp32", False), ("f32", "fp32", True),
                                                 ("u10", "split_bf16", False)])
 def test_non_separable_window(fmt, precision, asym):
