@@ -10,15 +10,18 @@
 //      D2 = WY * (slice * WX), in a real cos/sin basis (16 real values per axis); the
 //      output of the first MFMA is already the B operand of the second, so no transpose
 //      is needed.  FP32 MFMA, except that 8-bit samples, exact in BF16, meet the exact
-//      three-part BF16 split of WX on the BF16 MFMA (exact products, FP32 sums);
+//      three-part BF16 split of WX on the BF16 MFMA (exact products, FP32 sums), and
+//      9- to 16-bit samples enter as their two bytes on the 16x16x32 BF16 MFMA;
 //   3. forms the temporal real basis in registers, stages the T*256 values in LDS and
 //      regroups them so that each lane owns one (y pair, x pair) group: from its 4*T
 //      values it forms its 2*T complex coefficients X[kt][ky][kx], applies zero-mean,
 //      the gain and the mean re-add (cpu_source/kernel.hpp:1819-1897), maps them back
 //      to the real basis and evaluates the inverse over t at the centre slice;
 //   4. runs the inverse real DFT (C2R convention, cpu_source/kernel.hpp:1978-1982) as
-//      two chained MFMA contractions, multiplies by the synthesis window and overlap-
-//      adds into the tile accumulator in LDS (cpu_source/getframe_impl.cpp:158-175).
+//      two chained MFMA contractions, multiplies by the synthesis window and stages the
+//      block output in LDS; after each pass of 8 blocks the pixels' owning threads add
+//      the staged outputs covering them to the tile accumulator in LDS, in a fixed order
+//      and without atomics (cpu_source/getframe_impl.cpp:158-175).
 // The finished tile region goes to a scratch buffer; the finalize kernel sums the at
 // most 4 tile regions covering each pixel and converts to the sample type with the
 // reference rounding (cpu_source/getframe_impl.cpp:177-221).
