@@ -172,26 +172,30 @@ def test_frame_entry_host_and_device_match_batch():
         np.testing.assert_array_equal(dout.cpu().numpy(), full[fr])
 
 
-def test_frame_entry_pinned_host_planes():
+@pytest.mark.parametrize("fmt", ["u8", "u10", "f32"])
+def test_frame_entry_pinned_host_planes(fmt):
     # Pinned host planes take the direct copies, pageable ones the pinned staging buffer:
-    # both give the batch result, also with row strides wider than the plane.
-    announce("process_frame pinned host planes")
+    # both give the batch result, also with row strides wider than the plane (strides in
+    # bytes, samples of 1, 2 and 4 bytes).
+    announce("process_frame pinned host planes", fmt)
     p = dfttest_params()
-    x, bits, _ = make_plane("u8", (5, 46, 58), 43)
-    full = gpu_filter(x, p, bits, False)
-    plan = get_plan(p, "int", 8, Backend.MI355X())
+    x, bits, is_float = make_plane(fmt, (5, 46, 58), 43)
+    full = gpu_filter(x, p, bits, is_float)
+    plan = get_plan(p, "float" if is_float else "int", bits, Backend.MI355X())
     n, h, w = x.shape
     pitch = 64
-    wide = torch.zeros((n, h, pitch), dtype=torch.uint8).pin_memory()
-    wide[:, :, :w] = torch.from_numpy(x)
+    tdt = {np.dtype(np.uint8): torch.uint8, np.dtype(np.uint16): torch.int16, np.dtype(np.float32): torch.float32}
+    wide = torch.zeros((n, h, pitch), dtype=tdt[x.dtype]).pin_memory()
+    wide[:, :, :w] = torch.from_numpy(x.view(np.int16) if x.dtype == np.uint16 else x)
+    stride = pitch * x.itemsize
     for fr in (0, 2, 4):
         idx = [min(max(fr - 1 + t, 0), n - 1) for t in range(3)]
-        out = torch.zeros((h, pitch), dtype=torch.uint8).pin_memory()
-        plan.process_frame([wide[i].data_ptr() for i in idx], pitch, out.data_ptr(), pitch, w, h, memory=1)
-        np.testing.assert_array_equal(out[:, :w].numpy(), full[fr])
-        paged = np.zeros((h, pitch), np.uint8)
-        srcs = [np.ascontiguousarray(wide[i].numpy()) for i in idx]
-        plan.process_frame([s_.ctypes.data for s_ in srcs], pitch, paged.ctypes.data, pitch, w, h, memory=1)
+        out = torch.zeros((h, pitch), dtype=wide.dtype).pin_memory()
+        plan.process_frame([wide[i].data_ptr() for i in idx], stride, out.data_ptr(), stride, w, h, memory=1)
+        np.testing.assert_array_equal(out[:, :w].numpy().view(x.dtype), full[fr])
+        paged = np.zeros((h, pitch), x.dtype)
+        srcs = [np.ascontiguousarray(wide[i].numpy().view(x.dtype)) for i in idx]
+        plan.process_frame([s_.ctypes.data for s_ in srcs], stride, paged.ctypes.data, stride, w, h, memory=1)
         np.testing.assert_array_equal(paged[:, :w], full[fr])
 
 
