@@ -55,7 +55,14 @@ extern "C" {
 #define DFT2_PRECISION_FP32 0      /* FP32 accuracy: FP32 MFMA; with 8-bit samples the x
                                     * transform runs on the BF16 MFMA over an exact
                                     * three-part split of its basis (exact products) */
-#define DFT2_PRECISION_SPLIT_BF16 1 /* hi+lo BF16 operands, three products, FP32 accumulate */
+#define DFT2_PRECISION_SPLIT_BF16 1 /* hi+lo BF16 operands, three products, FP32 accumulate.
+                                    * Coefficient error about 2^-15 of the block's L1 norm
+                                    * (FP32: 2^-18).  Filter types 0, 1, 2, 5 and 6 only:
+                                    * plans with types 3 and 4, whose gains exceed 1 and (type
+                                    * 3) jump at pmin/pmax, are rejected.  Type 2 scales the
+                                    * error by sigma.  Type 1 decides psd < sigma within that
+                                    * error: near-threshold coefficients may be kept or
+                                    * dropped otherwise than in FP64 (as in FP32, more often). */
 
 /* Stream argument naming the null (default) stream: the work is queued there and not
  * waited for (NULL instead selects the plan's own stream and a blocking call).  Same
@@ -157,6 +164,20 @@ DFT2_API int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_job
 DFT2_API int dft2_process_batch_sigma(dft2_plan* plan, const dft2_job* jobs, int n_jobs, const float* sigma_dev,
                              float sigma_scale, void* stream);
 
+/* Parity instrumentation for the hard-threshold filters: as dft2_process_batch for a plan
+ * with filter_type 1 (either precision) or 3 (FP32) and a symmetric sigma, and also writes every block's gain decisions to decisions_dev (device memory,
+ * decisions_len 64-bit entries).  Layout: for each job in order, each output frame
+ * [frame_begin, frame_end), each block row by < pad_num(height) and column bx <
+ * pad_num(width) (cpu_source/getframe_impl.cpp:18-20), (radius+1)*4 masks [g][slot];
+ * bit `lane` (0..63, lane = 8p + q, slot = 2e + k) is the decision on coefficient
+ * (kt, ky, kx): for q != 0, ky = (p or 0 when p = 0) if e = 0, else (16 - p or 8 when
+ * p = 0), and (kt, ky, kx) = (g, ky, q) if k = 0, ((T - g) % T, (16 - ky) % 16, q) if k = 1;
+ * for q = 0, (g, ky, 0) if k = 0 and (g, ky, 8) if k = 1.  The decision of type 1 is
+ * psd < sigma (dropped), of type 3 pmin <= psd <= pmax (sigma applies).  No reference
+ * counterpart. */
+DFT2_API int dft2_process_batch_decisions(dft2_plan* plan, const dft2_job* jobs, int n_jobs,
+                                          unsigned long long* decisions_dev, long long decisions_len, void* stream);
+
 /* Backward pass of dft2_process_batch_sigma for the Wiener filter (filter_type 0) on
  * float32 planes: accumulates d loss / d src into each job's grad_in and
  * d loss / d sigma_dev into sigma_grad_dev (T*16*9 floats, device, +=), given grad_out.
@@ -174,9 +195,12 @@ typedef struct dft2_kernel_timing {
 } dft2_kernel_timing;
 
 /* enable = 1: from now on the plan's process calls record HIP events around every fused
- * and finalize launch, on the stream the launch uses (two extra event records per
- * launch); the totals start from zero.  enable = 0 stops recording.  No reference
- * counterpart: instrumentation for throughput and roofline measurement. */
+ * and finalize launch, on the stream the launch uses (three events per launch pair);
+ * the totals start from zero.  At most 64 launch pairs stay pending: a process call
+ * that would exceed that first waits for the oldest pair and adds it to the totals, so
+ * a long run holds a fixed number of events (and, with more than 64 launch pairs in
+ * flight, waits for the GPU).  enable = 0 stops recording.  No reference counterpart:
+ * instrumentation for throughput and roofline measurement. */
 DFT2_API int dft2_plan_set_timing(dft2_plan* plan, int enable);
 
 /* Waits for the recorded events and returns the totals since timing was enabled. */

@@ -89,12 +89,16 @@ def sample_scale(bits: int, is_float: bool) -> float:
     return 255.0 if is_float else 1.0 / (1 << (bits - 8))
 
 
-def filter_blocks(blocks: np.ndarray, p) -> np.ndarray:
+def filter_blocks(blocks: np.ndarray, p, decisions: np.ndarray = None) -> np.ndarray:
     """fused() on windowed blocks [..., T, B, B] -> centre-slice spatial output [..., B, B].
 
     cpu_source/kernel.hpp:1899-1983: unnormalised forward 3-D real DFT, zero-mean,
     gain, mean re-add, inverse over t evaluated at the centre slice, inverse over y,
     C2R over x (Im at kx = 0 and kx = B/2 ignored).
+    decisions: for filter types 1 and 3, boolean [..., T, B, B/2+1] replacing the
+    comparisons of psd with the thresholds (type 1: True = dropped, psd < sigma; type 3:
+    True = pmin <= psd <= pmax, gain sigma), so that another arithmetic's decisions can
+    be replayed; the gains are otherwise those of `gain`.
     """
     t_len, bs = p.t_len, p.block_size
     r = p.radius
@@ -107,7 +111,12 @@ def filter_blocks(blocks: np.ndarray, p) -> np.ndarray:
         mean = gf[..., None, None, None] * wf
         spec = spec - mean
     psd = spec.real ** 2 + spec.imag ** 2
-    g = gain(psd, sigma, float(f32(p.sigma2)), float(f32(p.pmin)), float(f32(p.pmax)), p.filter_type)
+    if decisions is not None and p.filter_type == 1:
+        g = np.where(decisions, 0.0, 1.0)
+    elif decisions is not None and p.filter_type == 3:
+        g = np.where(decisions, sigma, float(f32(p.sigma2)))
+    else:
+        g = gain(psd, sigma, float(f32(p.sigma2)), float(f32(p.pmin)), float(f32(p.pmax)), p.filter_type)
     spec = spec * g
     if p.zero_mean:
         spec = spec + mean
@@ -120,12 +129,46 @@ def filter_blocks(blocks: np.ndarray, p) -> np.ndarray:
     return out
 
 
+def _frame_blocks(padded: np.ndarray, n: int, p, scale: float, window: np.ndarray, ny: int, nx: int) -> np.ndarray:
+    """Windowed blocks [ny, nx, T, B, B] of output frame n: temporal clamp
+    (cpu_source/getframe_impl.cpp:294-298), block origins at block_step, b = scale * w * p
+    (cpu_source/getframe_impl.cpp:105-156)."""
+    n_frames = padded.shape[0]
+    bs, step, r = p.block_size, p.block_step, p.radius
+    idx = [min(max(n - r + t, 0), n_frames - 1) for t in range(p.t_len)]
+    slabs = []
+    for fi in idx:
+        view = np.lib.stride_tricks.sliding_window_view(padded[fi], (bs, bs))[::step, ::step]
+        slabs.append(view[:ny, :nx])
+    return np.stack(slabs, axis=2) * (scale * window)
+
+
+def _geometry(frames: np.ndarray, p):
+    _, h, w = frames.shape
+    bs, step = p.block_size, p.block_step
+    ph, pw = pad_size(h, bs, step), pad_size(w, bs, step)
+    ny, nx = pad_num(h, bs, step), pad_num(w, bs, step)
+    return ph, pw, ny, nx, (ph - h) // 2, (pw - w) // 2
+
+
+def _overlap_add(values: np.ndarray, ph: int, pw: int, step: int) -> np.ndarray:
+    """Sums per-block [ny, nx, B, B] values into the padded plane (block origins at step)."""
+    ny, nx, bs, _ = values.shape
+    acc = np.zeros((ph, pw))
+    for dy in range(bs):
+        for dx in range(bs):
+            acc[dy:dy + step * ny:step, dx:dx + step * nx:step] += values[:, :, dy, dx]
+    return acc
+
+
 def process_plane(frames: np.ndarray, p, bits: int, is_float: bool, syn_slice: str = "centre",
-                  return_acc: bool = False):
+                  return_acc: bool = False, decisions=None):
     """Filter one plane of a clip: frames [N, H, W] -> [N, H, W] (dtype of the input).
 
     Per output frame: temporal clamp, mirror pad, blocks at block_step, filter_blocks,
     overlap-add with the synthesis window, store (see module docstring).
+    decisions: per output frame, the [ny, nx, T, B, B/2+1] hard decisions filter_blocks
+    replays (filter types 1 and 3), or None.
     """
     n_frames, h, w = frames.shape
     bs, step, r = p.block_size, p.block_step, p.radius
@@ -134,24 +177,14 @@ def process_plane(frames: np.ndarray, p, bits: int, is_float: bool, syn_slice: s
     peak = (1 << bits) - 1
     window = f32(p.window).reshape(t_len, bs, bs)
     syn = window[r] if syn_slice == "centre" else window[2 * r]
-    ph, pw = pad_size(h, bs, step), pad_size(w, bs, step)
-    ny, nx = pad_num(h, bs, step), pad_num(w, bs, step)
-    oy, ox = (ph - h) // 2, (pw - w) // 2
+    ph, pw, ny, nx, oy, ox = _geometry(frames, p)
     padded = np.stack([mirror_pad(frames[i].astype(np.float64), bs, step) for i in range(n_frames)])
     out = np.empty(frames.shape, dtype=frames.dtype)
     accs = []
     for n in range(n_frames):
-        idx = [min(max(n - r + t, 0), n_frames - 1) for t in range(t_len)]
-        slabs = []
-        for t, fi in enumerate(idx):
-            view = np.lib.stride_tricks.sliding_window_view(padded[fi], (bs, bs))[::step, ::step]
-            slabs.append(view[:ny, :nx])
-        blocks = np.stack(slabs, axis=2) * (scale * window)  # [ny, nx, T, B, B]
-        yc = filter_blocks(blocks, p) * syn  # [ny, nx, B, B]
-        acc = np.zeros((ph, pw))
-        for dy in range(bs):
-            for dx in range(bs):
-                acc[dy:dy + step * ny:step, dx:dx + step * nx:step] += yc[:, :, dy, dx]
+        blocks = _frame_blocks(padded, n, p, scale, window, ny, nx)  # [ny, nx, T, B, B]
+        yc = filter_blocks(blocks, p, None if decisions is None else decisions[n]) * syn  # [ny, nx, B, B]
+        acc = _overlap_add(yc, ph, pw, step)
         core = acc[oy:oy + h, ox:ox + w]
         if return_acc:
             accs.append(core.copy())
@@ -163,3 +196,63 @@ def process_plane(frames: np.ndarray, p, bits: int, is_float: bool, syn_slice: s
     if return_acc:
         return out, np.stack(accs) / scale
     return out
+
+
+# Error bound of one block DFT coefficient as the GPU forms compute it, as a multiple of
+# the block's L1 norm sum|b| (which bounds every coefficient).  With u = 2^-24 (FP32) and
+# u' = 2^-9 (BF16):
+#   fp32: each of the forward stages (x contraction with exact products or FP32 MFMA, y
+#     contraction on the FP32 MFMA, T-term temporal sums) adds at most (n - 1) u of the
+#     sum of its |terms|, and the FP32 bases add u each: about 40 u = 2^-18.7 < 2^-18.
+#   split_bf16: a value a = hi + lo + e with |e| <= u'^2 |a| = 2^-18 |a|, and a product
+#     drops lo_a lo_b (<= 2^-18 |ab|): at most 3 * 2^-18 per split contraction, two
+#     contractions, plus the FP32 sums: 6 * 2^-18 + 40 u = 2^-15.3 < 2^-15.
+COEF_ERROR_BOUND = {"fp32": 2.0 ** -18, "split_bf16": 2.0 ** -15}
+
+
+def hard_decisions(frames: np.ndarray, p, bits: int, is_float: bool, kappa: float):
+    """FP64 hard decisions of filter types 1 and 3 and where another arithmetic may differ.
+
+    Returns, per output frame, (decisions, near, closeness, psd, margin), each
+    [ny, nx, T, B, B/2+1]:
+    decisions as filter_blocks takes them (type 1: psd < sigma; type 3: pmin <= psd <=
+    pmax, cpu_source/kernel.hpp:1826-1890), with psd = |X'|^2 after the zero-mean step;
+    near: the coefficient's psd lies within 2 |X'| d + d^2 of a threshold, where d bounds
+    the coefficient error of a form whose error is at most kappa * sum|b| per block
+    (COEF_ERROR_BOUND), times 1 + |WF|/|WF0| for the zero-mean step (it subtracts gf * WF
+    with gf = X0/WF0); closeness: |psd - threshold| over that margin (<= 1 where near);
+    psd and margin themselves.  Only near coefficients can be decided otherwise by that
+    form.
+    """
+    n_frames = frames.shape[0]
+    if p.filter_type not in (1, 3):
+        raise ValueError("hard decisions exist for filter types 1 and 3")
+    bs, step, t_len = p.block_size, p.block_step, p.t_len
+    scale = sample_scale(bits, is_float)
+    window = f32(p.window).reshape(t_len, bs, bs)
+    ph, pw, ny, nx, oy, ox = _geometry(frames, p)
+    padded = np.stack([mirror_pad(frames[i].astype(np.float64), bs, step) for i in range(n_frames)])
+    sigma = f32(p.sigma_array())
+    res = []
+    for n in range(n_frames):
+        blocks = _frame_blocks(padded, n, p, scale, window, ny, nx)
+        spec = np.fft.rfftn(blocks, axes=(-3, -2, -1))
+        d = kappa * np.abs(blocks).sum(axis=(-3, -2, -1))[..., None, None, None]
+        if p.zero_mean:
+            wf = f32(p.window_freq).reshape(t_len, bs, bs // 2 + 1, 2)
+            wf = wf[..., 0] + 1j * wf[..., 1]
+            spec = spec - (spec[..., 0, 0, 0].real / wf[0, 0, 0].real)[..., None, None, None] * wf
+            d = d * (1.0 + np.abs(wf) / abs(wf[0, 0, 0].real))
+        mag = np.abs(spec)
+        psd = mag * mag
+        margin = 2.0 * mag * d + d * d
+        if p.filter_type == 1:
+            dec = psd < sigma
+            dist = np.abs(psd - sigma)
+        else:
+            lo, hi = float(f32(p.pmin)), float(f32(p.pmax))
+            dec = (psd >= lo) & (psd <= hi)
+            dist = np.minimum(np.abs(psd - lo), np.abs(psd - hi))
+        closeness = dist / np.maximum(margin, 1e-300)
+        res.append((dec, closeness <= 1.0, closeness, psd, margin))
+    return res

@@ -353,10 +353,18 @@ SPLIT_CASES = [
     ("u8", (5, 64, 80), dict()),
     ("u10", (6, 60, 72), dict(ftype=1, tbsize=5, slocation=[0.0, 4, 0.5, 8, 1.0, 16])),
     ("u8", (8, 50, 66), dict(tbsize=7, swin=1)),
-    ("u12", (4, 48, 48), dict(tbsize=1, ftype=3, pmax=1e4)),
-    ("u16", (4, 48, 48), dict(tbsize=1, ftype=3, pmax=1e4)),
+    ("u12", (4, 48, 48), dict(tbsize=1, f0beta=0.5)),
+    ("u16", (4, 48, 48), dict(tbsize=1, ftype=2, sigma=0.8)),
     ("f32", (5, 55, 64), dict()),
 ]
+
+
+@pytest.mark.parametrize("ftype", [3, 4])
+def test_split_bf16_rejects_gains_above_one(ftype):
+    announce("split-bf16 plan rejects filter type", ftype)
+    p = dfttest_params(ftype=ftype, sigma=4.0, pmin=1.0, pmax=100.0)
+    with pytest.raises(_lib.Dft2Error, match="split BF16"):
+        get_plan(p, "int", 10, Backend.MI355X(precision="split_bf16"))
 
 
 @pytest.mark.parametrize("fmt,shape,kw", SPLIT_CASES, ids=[f"{c[0]}-{i}" for i, c in enumerate(SPLIT_CASES)])

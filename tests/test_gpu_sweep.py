@@ -2,27 +2,29 @@
 
 Each case draws a sample format, precision, tbsize, sosize, window pair, zero-mean, filter
 type and its parameters, and a small clip shape, then compares the plugin-path output with
-oracle/dfttest_np.  What is asserted follows the filter type:
+oracle/dfttest_np on every sample.  Bars: FP32 1 LSB (integers) and 1e-5 (float32);
+split BF16 1 LSB up to 12 bits, 4 LSB at 16 bits, 5e-5 for float32, times sigma for the
+fixed-gain filter (type 2 multiplies the coefficients, and their error, by sigma).
 
-* continuous gains (filter types 0, 2, 4, 5, 6): the FP32 form within the parity bars (1 LSB,
-  1e-5 for float32); the split-BF16 form within its bars for types 0, 2, 5, 6;
-* hard decisions (type 1: psd against sigma; type 3: sigma or sigma2 by pmin <= psd <= pmax):
-  a coefficient whose psd lies within the arithmetic's error of a threshold is kept or
-  dropped differently than in FP64 (the reference's own FP32 arithmetic is exposed the same
-  way), so the FP32 form is held to at least 95 % of samples within the bar (the share a
-  flip touches grows with the overlap: 1.7 % at block step 2 in this sweep; the reference's
-  own golden vectors for these types stay within 1 LSB).  The split-BF16
-  form's absolute error, relative to the block's largest coefficients, flips such decisions
-  far more often, and with type 4 (whose gain is steep near pmin) exceeds the bar on a small
-  share of samples: those cases only have to run.
+* Continuous gains (filter types 0, 2, 4, 5, 6): every sample within the bar of FP64.
+* Hard decisions (type 1: psd < sigma drops a coefficient; type 3: sigma or sigma2 by
+  pmin <= psd <= pmax): the kernel records every block's decisions
+  (dft2_process_batch_decisions).  Every decision that differs from FP64 must be on a
+  coefficient whose FP64 psd lies within the form's error margin of its threshold
+  (oracle hard_decisions), and the FP64 oracle replaying the kernel's own decisions must
+  match the output on every sample within the bar.  The share of samples that differ from
+  plain FP64 by more than the bar (those near-threshold decisions) is printed.
+* Split BF16 with types 3 and 4 (gains above 1, type 3 jumping between them) is rejected
+  at plan creation: the case checks the error.
 """
 
 import numpy as np
 import pytest
 
-from dfttest2_amd import synth
-from dfttest2_amd.dfttest2 import Backend, VideoClip, filter_clip
+from dfttest2_amd import _lib, synth
+from dfttest2_amd.dfttest2 import Backend, VideoClip, filter_clip, get_plan
 from dfttest2_amd.params import dfttest_params
+from decision_util import run_with_decisions
 from oracle import dfttest_np as onp
 
 pytestmark = pytest.mark.gpu
@@ -52,6 +54,41 @@ def draw_case(rng, i):
 CASES = [draw_case(np.random.default_rng(5150 + i), i) for i in range(120)]
 
 
+def bar_of(prec, bits, is_float, p):
+    if prec == "fp32":
+        return 1e-5 if is_float else 1.0
+    bar = 5e-5 if is_float else (4.0 if bits > 12 else 1.0)
+    if p.filter_type == 2:
+        bar *= max(1.0, float(np.max(np.abs(p.sigma))))
+    return bar
+
+
+def check_hard_decisions(x, p, bits, is_float, prec, bar, label):
+    """Decision-replay check of one clip (filter types 1 and 3); returns the output."""
+    plan = get_plan(p, "float" if is_float else "int", bits, Backend.MI355X(precision=prec))
+    _, _, ny, nx, _, _ = onp._geometry(x, p)
+    out, dec_gpu = run_with_decisions(plan, x, p, (ny, nx))
+    plain = filter_clip(VideoClip([x], "float" if is_float else "int", bits), p,
+                        Backend.MI355X(precision=prec)).planes[0]
+    np.testing.assert_array_equal(out, plain)  # recording leaves the arithmetic unchanged
+    fp64 = onp.hard_decisions(x, p, bits, is_float, onp.COEF_ERROR_BOUND[prec])
+    flips, worst_close = 0, 0.0
+    for n, ((dec, near, close, _, _), dg) in enumerate(zip(fp64, dec_gpu)):
+        miss = dec != dg
+        flips += int(miss.sum())
+        if miss.any():
+            worst_close = max(worst_close, float(close[miss].max()))
+        assert not np.any(miss & ~near), f"frame {n}: a decision outside the {prec} error margin differs from FP64"
+    replay = onp.process_plane(x, p, bits, is_float, decisions=dec_gpu)
+    d_replay = np.abs(out.astype(np.float64) - replay.astype(np.float64))
+    d_fp64 = np.abs(out.astype(np.float64) - onp.process_plane(x, p, bits, is_float).astype(np.float64))
+    print(f"[{label}] decisions differing from FP64: {flips} (largest |psd - threshold| / margin "
+          f"{worst_close:.3g}); samples beyond the bar vs plain FP64: {np.mean(d_fp64 > bar):.4%} "
+          f"(max {d_fp64.max():.3g}); vs replay max {d_replay.max():.3g}", flush=True)
+    assert d_replay.max() <= bar
+    return out
+
+
 @pytest.mark.parametrize("case", range(len(CASES)))
 def test_random_parameters(case):
     fmt, prec, kw, shape = CASES[case]
@@ -61,24 +98,35 @@ def test_random_parameters(case):
     p = dfttest_params(**kw)
     if p.zero_mean and abs(float(np.asarray(p.window_freq)[0])) == 0.0:
         pytest.skip("window spectrum DC is 0: zero-mean is undefined (the plan rejects it)")
+    if prec == "split_bf16" and p.filter_type in (3, 4):
+        with pytest.raises(_lib.Dft2Error, match="split BF16"):
+            get_plan(p, "float" if is_float else "int", bits, Backend.MI355X(precision=prec))
+        return
     n, h, w = shape
     if is_float:
         x = synth.float_plane(n, h, w, 300 + case, 0.05).numpy()
     else:
         x = synth.int_plane(n, h, w, 300 + case, bits, {8: 6.0, 10: 12.0, 12: 50.0, 16: 700.0}[bits]).numpy()
+    bar = bar_of(prec, bits, is_float, p)
+    if p.filter_type in (1, 3):
+        check_hard_decisions(x, p, bits, is_float, prec, bar, f"sweep {case}")
+        return
     out = filter_clip(VideoClip([x], "float" if is_float else "int", bits), p, Backend.MI355X(precision=prec))
     ref = onp.process_plane(x, p, bits, is_float)
-    diff = np.abs(out.planes[0].astype(np.float64) - ref.astype(np.float64))
-    if prec == "fp32":
-        bar = 1e-5 if is_float else 1.0
-    else:
-        bar = 5e-5 if is_float else (4.0 if bits > 12 else 1.0)
-    hard = p.filter_type in (1, 3)
-    if prec == "fp32" and not hard:
-        assert diff.max() <= bar
-    elif prec == "fp32":
-        assert np.mean(diff > bar) <= 0.05, f"max {diff.max():.3g}"
-    elif p.filter_type in (0, 2, 5, 6):
-        assert diff.max() <= bar
-    else:
-        assert np.isfinite(out.planes[0].astype(np.float64)).all()
+    assert np.abs(out.planes[0].astype(np.float64) - ref.astype(np.float64)).max() <= bar
+
+
+# The precision workload's filter (YUV420P10, ftype 1, tbsize 5, sigma curve 0.0:4 0.5:8
+# 1.0:16) at more seeds and sizes, both precisions, by decision replay.
+CONFIG_PRECISION_CASES = [(prec, shape, seed) for prec in ("fp32", "split_bf16")
+                          for shape, seed in (((6, 120, 160), 901), ((4, 200, 264), 902), ((3, 360, 640), 903),
+                                              ((8, 90, 130), 904))]
+
+
+@pytest.mark.parametrize("prec,shape,seed", CONFIG_PRECISION_CASES)
+def test_precision_workload_filter_by_decision_replay(prec, shape, seed):
+    print(f"\n[case] u10 ftype 1 tbsize 5 sigma curve {prec} {shape} seed {seed}", flush=True)
+    p = dfttest_params(ftype=1, tbsize=5, slocation=[0.0, 4, 0.5, 8, 1.0, 16])
+    n, h, w = shape
+    x = synth.int_plane(n, h, w, seed, 10, 12.0).numpy()
+    check_hard_decisions(x, p, 10, False, prec, 1.0, f"{prec} {shape}")

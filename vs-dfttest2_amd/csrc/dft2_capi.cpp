@@ -365,6 +365,7 @@ LaunchArgs make_args(const dft2_plan* p, const TileShape& ts) {
     a.tab_dev = nullptr;
     a.queue = nullptr;
     a.n_items = 0;
+    a.decisions = nullptr;
     return a;
 }
 
@@ -446,6 +447,7 @@ int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDe
             const int cnt = std::max(0, std::min<int>(static_cast<int>(frames_per_chunk), j.f_count - f0));
             j.f_begin = j.f_begin + f0;
             j.f_count = cnt;
+            j.dec_base += static_cast<long long>(f0) * j.nby * j.nbx;
             j.item_begin = ch.items;
             j.frame_begin = ch.frames;
             ch.items += static_cast<long long>(cnt) * j.tiles_x * j.tiles_y;
@@ -459,22 +461,58 @@ int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDe
     return DFT2_OK;
 }
 
-// With timing enabled, hands out the next three events of the plan's pool (creating
-// them on first use); otherwise leaves ev[] null.
+// With timing enabled, hands out three events from the plan's free pool (creating them
+// when it is empty); otherwise leaves ev[] null.  The caller records them around its
+// launch pair and then passes them to timing_commit.
 int timing_events(dft2_plan* p, hipEvent_t ev[3]) {
     std::lock_guard<std::mutex> lk(p->mu);
     if (!p->timing) {
         return DFT2_OK;
     }
-    while (p->timing_events.size() < p->timing_used + 3) {
-        hipEvent_t e = nullptr;
-        HIP_TRY(hipEventCreate(&e));
-        p->timing_events.push_back(e);
-    }
     for (int i = 0; i < 3; i++) {
-        ev[i] = p->timing_events[p->timing_used + i];
+        if (p->timing_free.empty()) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreate(&e));
+            p->timing_free.push_back(e);
+        }
+        ev[i] = p->timing_free.back();
+        p->timing_free.pop_back();
     }
-    p->timing_used += 3;
+    return DFT2_OK;
+}
+
+// Waits for the oldest pending triple, adds its durations to the totals and returns its
+// events to the free pool.  Called with p->mu held.
+int timing_resolve_oldest(dft2_plan* p) {
+    const std::array<hipEvent_t, 3> ev = p->timing_pending.front();
+    p->timing_pending.pop_front();
+    for (hipEvent_t e : ev) {
+        p->timing_free.push_back(e);
+    }
+    HIP_TRY(hipEventSynchronize(ev[2]));
+    float f = 0.0f, g = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&f, ev[0], ev[1]));
+    HIP_TRY(hipEventElapsedTime(&g, ev[1], ev[2]));
+    p->timing_totals.fused_launches++;
+    p->timing_totals.fused_ms += f;
+    p->timing_totals.finalize_launches++;
+    p->timing_totals.finalize_ms += g;
+    return DFT2_OK;
+}
+
+// Queues a recorded triple; beyond kTimingPending pending triples the oldest is resolved.
+int timing_commit(dft2_plan* p, const hipEvent_t ev[3]) {
+    if (!ev[0]) {
+        return DFT2_OK;
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    p->timing_pending.push_back({ev[0], ev[1], ev[2]});
+    while (p->timing_pending.size() > kTimingPending) {
+        int rc = timing_resolve_oldest(p);
+        if (rc) {
+            return rc;
+        }
+    }
     return DFT2_OK;
 }
 
@@ -496,7 +534,8 @@ int build_gain_table(const dft2_plan* p, Slot* slot, LaunchArgs& a, bool bias, h
 
 // Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
 int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
-             const float* sigma_dev = nullptr, float sigma_scale = 1.0f) {
+             const float* sigma_dev = nullptr, float sigma_scale = 1.0f,
+             unsigned long long* decisions = nullptr) {
     const TileShape& ts = p->tile;
     ChunkPlan cp;
     int rc = plan_chunks(p, ts, jobs_in, p->scratch_cap, cp);
@@ -526,7 +565,11 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
             return rc;
         }
     }
-    const FusedKey key = fused_key(p, 0, a.asym);
+    FusedKey key = fused_key(p, 0, a.asym);
+    if (decisions) {
+        a.decisions = decisions;
+        key.rec = 1;
+    }
     for (size_t c = 0; c < cp.chunks.size(); c++) {
         const Chunk& ch = cp.chunks[c];
         if (ch.items == 0) {
@@ -548,6 +591,10 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         HIP_TRY(launch_finalize(a, p->sample_code, ch.items, stream));
         if (ev[2]) {
             HIP_TRY(hipEventRecord(ev[2], stream));
+        }
+        rc = timing_commit(p, ev);
+        if (rc) {
+            return rc;
         }
     }
     return DFT2_OK;
@@ -648,6 +695,12 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
     }
     if (precision != DFT2_PRECISION_FP32 && precision != DFT2_PRECISION_SPLIT_BF16) {
         return fail(DFT2_ERR_ARG, "precision must be 0 (FP32) or 1 (split BF16)");
+    }
+    // Filter types 3 and 4 take gains above 1 (sigma, sigma2): the split form's coefficient
+    // error (about 2^-15 of the block's largest) is amplified with them, and type 3 jumps
+    // between its two gains at pmin and pmax, so that form would miss the output bars.
+    if (precision == DFT2_PRECISION_SPLIT_BF16 && (prm->filter_type == 3 || prm->filter_type == 4)) {
+        return fail(DFT2_ERR_ARG, "split BF16 precision supports filter types 0, 1, 2, 5 and 6 (use FP32 for types 3 and 4)");
     }
     const int T = 2 * prm->radius + 1;
     if (!prm->window || prm->window_len != T * 256) {
@@ -752,11 +805,14 @@ int dft2_plan_set_timing(dft2_plan* p, int enable) {
     }
     std::lock_guard<std::mutex> lk(p->mu);
     if (enable) {
-        // Events handed out earlier may still be pending; wait before they are reused.
-        for (size_t i = 0; i < p->timing_used; i++) {
-            HIP_TRY(hipEventSynchronize(p->timing_events[i]));
+        // Pending events are waited for and dropped; the totals start from zero.
+        while (!p->timing_pending.empty()) {
+            int rc = timing_resolve_oldest(p);
+            if (rc) {
+                return rc;
+            }
         }
-        p->timing_used = 0;
+        p->timing_totals = dft2_kernel_timing{};
     }
     p->timing = enable != 0;
     return DFT2_OK;
@@ -768,17 +824,13 @@ int dft2_plan_get_timing(dft2_plan* p, dft2_kernel_timing* out) {
         return fail(DFT2_ERR_ARG, "plan and out must not be NULL");
     }
     std::lock_guard<std::mutex> lk(p->mu);
-    std::memset(out, 0, sizeof(*out));
-    for (size_t i = 0; i + 2 < p->timing_used; i += 3) {
-        HIP_TRY(hipEventSynchronize(p->timing_events[i + 2]));
-        float f = 0.0f, g = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&f, p->timing_events[i], p->timing_events[i + 1]));
-        HIP_TRY(hipEventElapsedTime(&g, p->timing_events[i + 1], p->timing_events[i + 2]));
-        out->fused_launches++;
-        out->fused_ms += f;
-        out->finalize_launches++;
-        out->finalize_ms += g;
+    while (!p->timing_pending.empty()) {
+        int rc = timing_resolve_oldest(p);
+        if (rc) {
+            return rc;
+        }
     }
+    *out = p->timing_totals;
     return DFT2_OK;
 }
 
@@ -825,8 +877,13 @@ int dft2_plan_destroy(dft2_plan* p) {
             (void)hipStreamDestroy(s.stream);
         }
     }
-    for (hipEvent_t e : p->timing_events) {
-        (void)hipEventSynchronize(e);
+    for (const auto& tr : p->timing_pending) {
+        for (hipEvent_t e : tr) {
+            (void)hipEventSynchronize(e);
+            (void)hipEventDestroy(e);
+        }
+    }
+    for (hipEvent_t e : p->timing_free) {
         (void)hipEventDestroy(e);
     }
     if (p->d_consts) {
@@ -863,10 +920,61 @@ int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, con
     Slot* slot = acquire_slot(p);
     SlotGuard guard{p, slot};
     hipStream_t st = call_stream(stream, slot->stream);
-    if (stream) {
-        HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
-    }
+    // The slot's buffers (scratch, job tables, gain table) may still be in use by its
+    // previous call, queued on another stream: order this call after it.
+    HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     rc = run_jobs(p, slot, jd, st, sigma_dev, sigma_scale);
+    if (rc) {
+        return rc;
+    }
+    HIP_TRY(hipEventRecord(slot->done, st));
+    if (!stream) {
+        HIP_TRY(hipStreamSynchronize(st));
+    }
+    return DFT2_OK;
+}
+
+int dft2_process_batch_decisions(dft2_plan* p, const dft2_job* jobs, int n_jobs, unsigned long long* decisions_dev,
+                                 long long decisions_len, void* stream) {
+    clear_error();
+    if (!p || !decisions_dev) {
+        return fail(DFT2_ERR_ARG, "plan and decisions_dev must not be NULL");
+    }
+    if (p->filter_type != 1 && p->filter_type != 3) {
+        return fail(DFT2_ERR_ARG, "decision records exist for filter types 1 and 3 only");
+    }
+    if (p->asym) {
+        return fail(DFT2_ERR_ARG, "decision records need a symmetric sigma");
+    }
+    if (p->filter_type == 3 && p->precision != DFT2_PRECISION_FP32) {
+        return fail(DFT2_ERR_ARG, "decision records of filter type 3 need FP32 precision");
+    }
+    if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
+        return fail(DFT2_ERR_ARG, "invalid job list");
+    }
+    std::vector<JobDev> jd(n_jobs);
+    long long blocks = 0;
+    for (int i = 0; i < n_jobs; i++) {
+        int rc = make_job(p, jobs[i], p->tile, jd[i]);
+        if (rc) {
+            return rc;
+        }
+        jd[i].dec_base = blocks;
+        blocks += static_cast<long long>(jd[i].f_count) * jd[i].nby * jd[i].nbx;
+    }
+    const long long need = blocks * (p->radius + 1) * 4;
+    if (decisions_len < need) {
+        return fail(DFT2_ERR_ARG, "decisions_dev holds %lld masks, %lld needed", decisions_len, need);
+    }
+    int rc = select_device(p);
+    if (rc) {
+        return rc;
+    }
+    Slot* slot = acquire_slot(p);
+    SlotGuard guard{p, slot};
+    hipStream_t st = call_stream(stream, slot->stream);
+    HIP_TRY(hipStreamWaitEvent(st, slot->done, 0)); // the slot's previous call, on any stream
+    rc = run_jobs(p, slot, jd, st, nullptr, 1.0f, decisions_dev);
     if (rc) {
         return rc;
     }
@@ -955,9 +1063,9 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     Slot* slot = acquire_slot(p);
     SlotGuard guard{p, slot};
     hipStream_t st = call_stream(stream, slot->stream);
-    if (stream) {
-        HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
-    }
+    // The slot's buffers (scratch, job tables, gain table) may still be in use by its
+    // previous call, queued on another stream: order this call after it.
+    HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     JobDev* d_table = nullptr;
     rc = upload_jobs(slot, jd.data(), sizeof(JobDev) * jd.size(), st, &d_table);
     if (rc) {
@@ -999,6 +1107,28 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         }
     }
     const int bps = p->bytes_per_sample;
+    if (memory != 0 && memory != 1) {
+        return fail(DFT2_ERR_ARG, "memory must be 0 (device) or 1 (host)");
+    }
+    // The caller's geometry, checked before anything is staged or copied (the host path
+    // copies height rows of width samples at the caller's strides).
+    if (width < 1 || height < 1) {
+        return fail(DFT2_ERR_ARG, "width and height must be positive (got %d x %d)", width, height);
+    }
+    if (src_stride % bps || dst_stride % bps) {
+        return fail(DFT2_ERR_ARG, "strides and pitches must be multiples of the sample size (%d)", bps);
+    }
+    if (src_stride < static_cast<long long>(width) * bps || dst_stride < static_cast<long long>(width) * bps) {
+        return fail(DFT2_ERR_ARG, "row stride smaller than the row");
+    }
+    int rc = check_mirror(width, p->block_step, "width");
+    if (rc) {
+        return rc;
+    }
+    rc = check_mirror(height, p->block_step, "height");
+    if (rc) {
+        return rc;
+    }
     dft2_job uj{};
     uj.src = src_frames[0];
     uj.dst = dst;
@@ -1010,7 +1140,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     uj.src_stride = src_stride;
     uj.dst_stride = dst_stride;
     JobDev j;
-    int rc = select_device(p);
+    rc = select_device(p);
     if (rc) {
         return rc;
     }
@@ -1028,9 +1158,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         Slot* slot = acquire_slot(p);
         SlotGuard guard{p, slot};
         hipStream_t st = call_stream(stream, slot->stream);
-        if (stream) {
-            HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
-        }
+        HIP_TRY(hipStreamWaitEvent(st, slot->done, 0)); // the slot's previous call, on any stream
         rc = run_jobs(p, slot, {j}, st);
         if (rc) {
             return rc;
@@ -1040,9 +1168,6 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
             HIP_TRY(hipStreamSynchronize(st));
         }
         return DFT2_OK;
-    }
-    if (memory != 1) {
-        return fail(DFT2_ERR_ARG, "memory must be 0 (device) or 1 (host)");
     }
     // Host memory: stage T planes and the output through device memory (the reference
     // copies T padded planes host->device per frame, hiprtc_source/source.cpp:570-573).
@@ -1060,6 +1185,9 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     Slot* slot = acquire_slot(p);
     SlotGuard guard{p, slot};
     hipStream_t st = slot->stream;
+    // The slot's previous call may have run on a caller's stream: wait for it before the
+    // staging buffers and scratch are reused.
+    HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
     HIP_TRY(hipStreamSynchronize(st));
     rc = ensure_device(&slot->stage, &slot->stage_bytes, static_cast<size_t>(plane) * (T + 1));
     if (rc) {

@@ -465,8 +465,11 @@ __device__ __forceinline__ float damping_of(float psd, float s) {
 // ASYM: 0 = symmetric sigma (no partner gains), 1 = asymmetric, 2 = decided at run time
 // from a.asym.  The forms are separate kernel instantiations because the compiler turns
 // a run-time test into selects and computes the partner gains for every plan.
+// Returns the coefficient's hard decision: filter type 1, psd < sigma (dropped); type 3
+// (FTM 2), pmin <= psd <= pmax; false otherwise.  Callers that do not record decisions
+// ignore it, and the compiler drops the comparison.
 template <int FTM, int ASYM>
-__device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, float gf, const LaunchArgs& a) {
+__device__ __forceinline__ bool filter_coef(float& re, float& im, float4 e, float gf, const LaunchArgs& a) {
     const float xr = fmaf(-gf, e.z, re);
     const float xi = fmaf(-gf, e.w, im);
     // The Wiener gain takes psd + 1e-15: the constant rides in the psd's own fma chain.
@@ -481,6 +484,7 @@ __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, floa
         }
         re = fmaf(-xr, h, re);
         im = fmaf(-xi, h, im);
+        return FTM == 1 && psd < e.x;
     } else {
         float g = gain_of<FTM>(psd, e.x, a);
         if constexpr (ASYM != 0) {
@@ -490,6 +494,7 @@ __device__ __forceinline__ void filter_coef(float& re, float& im, float4 e, floa
         }
         re = fmaf(xr, g, gf * e.z);
         im = fmaf(xi, g, gf * e.w);
+        return a.filter_type == 3 && psd >= a.pmin && psd <= a.pmax;
     }
 }
 

@@ -46,9 +46,11 @@ namespace {
 
 // Radius 0/1, Wiener or threshold gain, separable window: at most 80 VGPRs, so that six
 // waves per SIMD (three workgroups per CU) fit.
-template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP>
+// REC: also writes every block's gain decisions to a.decisions (parity instrumentation of
+// the hard-threshold filters; separate instantiations, so the others carry no trace of it).
+template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP, bool REC = false>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(R <= 1 && FTM != 2 && SEP && !PERSIST ? 6 : 1)))
+__attribute__((amdgpu_waves_per_eu(R <= 1 && FTM != 2 && SEP && !PERSIST && !REC ? 6 : 1)))
 void fused_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
     // Overlap-add: staged block outputs summed by the pixels' owners after a barrier, in a
@@ -388,6 +390,14 @@ void fused_kernel(LaunchArgs a) {
             }
 
             float zr[2][2], zi[2][2]; // [e][c] after the gains
+            // Decision record of this block at temporal pair g: [g][slot], slot = 2e + k.
+            unsigned long long* rec = nullptr;
+            if constexpr (REC) {
+                const long long blk = sgpr64(job.dec_base) +
+                                      (static_cast<long long>(f_local) * sgpr(job.nby) + by0 + r) * sgpr(job.nbx) +
+                                      bx0 + pass * kWaves + wave;
+                rec = a.decisions + (blk * (R + 1) + g) * 4;
+            }
 #pragma unroll
             for (int e = 0; e < 2; e++) {
                 const float z0r = e ? z2r[0] : z1r[0];
@@ -400,8 +410,16 @@ void fused_kernel(LaunchArgs a) {
                 float ai = fmaf(-exf, zxr, z0i);
                 float br = ex ? z0r - zxi : zxr;
                 float bi = ex ? -z0i - zxr : zxi;
-                filter_coef<FTM, ASYM>(ar, ai, tab_entry<ASYM>(&tab[(g * 4 + 2 * e) * 64 + lane]), gf, a);
-                filter_coef<FTM, ASYM>(br, bi, tab_entry<ASYM>(&tab[(g * 4 + 2 * e + 1) * 64 + lane]), gf, a);
+                const bool da = filter_coef<FTM, ASYM>(ar, ai, tab_entry<ASYM>(&tab[(g * 4 + 2 * e) * 64 + lane]), gf, a);
+                const bool db = filter_coef<FTM, ASYM>(br, bi, tab_entry<ASYM>(&tab[(g * 4 + 2 * e + 1) * 64 + lane]), gf, a);
+                if constexpr (REC) {
+                    const unsigned long long ma = __ballot(da);
+                    const unsigned long long mb = __ballot(db);
+                    if (lane == 0) {
+                        rec[2 * e] = ma;
+                        rec[2 * e + 1] = mb;
+                    }
+                }
                 // x slots back from (A, B), doubled for a proper pair (see gx above).
                 zr[e][0] = fmaf(exf, br, ar);
                 zi[e][0] = fmaf(-exf, bi, ai);
@@ -511,9 +529,23 @@ void fused_kernel(LaunchArgs a) {
 }
 
 template <int R, typename Tin, int PREC, bool PERSIST, bool SEP>
-const void* fused_ptr_ft(int filter_type, bool asym) {
+const void* fused_ptr_ft(int filter_type, bool asym, bool rec) {
     // Gains: 0 Wiener, 1 hard threshold (each with a symmetric and an asymmetric form),
-    // 2 the generic form for types 2..6.
+    // 2 the generic form for types 2..6.  Decision records exist for type 1 with a
+    // symmetric sigma and for type 3 in FP32, one item per workgroup.
+    if (rec) {
+        if constexpr (!PERSIST) {
+            if (filter_type == 1 && !asym) {
+                return reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, false, 0, SEP, true>);
+            }
+            if constexpr (PREC == 0) {
+                if (filter_type == 3) {
+                    return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, 0, false, 2, SEP, true>);
+                }
+            }
+        }
+        return nullptr;
+    }
     if (filter_type == 0) {
         return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 1, SEP>)
                     : reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, PERSIST, 0, SEP>);
@@ -533,15 +565,15 @@ const void* fused_ptr_prec(const FusedKey& k) {
         if (!k.sep) {
             return nullptr;
         }
-        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, true, true>(k.filter_type, k.asym != 0)
-                                : fused_ptr_ft<R, Tin, 0, true, true>(k.filter_type, k.asym != 0);
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, true, true>(k.filter_type, k.asym != 0, k.rec != 0)
+                                : fused_ptr_ft<R, Tin, 0, true, true>(k.filter_type, k.asym != 0, k.rec != 0);
     } else {
         if (k.sep) {
-            return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0)
-                                    : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0);
+            return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0, k.rec != 0)
+                                    : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0, k.rec != 0);
         }
-        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, false>(k.filter_type, k.asym != 0)
-                                : fused_ptr_ft<R, Tin, 0, false, false>(k.filter_type, k.asym != 0);
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, false>(k.filter_type, k.asym != 0, k.rec != 0)
+                                : fused_ptr_ft<R, Tin, 0, false, false>(k.filter_type, k.asym != 0, k.rec != 0);
     }
 }
 

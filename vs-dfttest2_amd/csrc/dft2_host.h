@@ -2,7 +2,9 @@
 // dft2_batch.cpp): the plan object, tile shapes, job validation and chunk planning.
 #pragma once
 
+#include <array>
 #include <condition_variable>
+#include <deque>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -33,6 +35,7 @@ int pad_num(int size, int bs, int step);
 
 constexpr int kRing = 8;
 constexpr int kTableCache = 4;
+constexpr size_t kTimingPending = 64; // launch pairs whose timing events may be pending
 
 // A job table resident in device memory, with the host bytes it was made from.
 struct DeviceTable {
@@ -114,10 +117,14 @@ struct dft2_plan {
     std::condition_variable cv;
     // Kernel timing (dft2_plan_set_timing): events before the fused launch, between the
     // fused and finalize launches and after the finalize launch, one triple per launch
-    // pair; guarded by mu.
+    // pair.  A triple is taken from the free pool before the launch and committed to the
+    // pending queue once recorded; at most kTimingPending triples stay pending (older ones
+    // are waited for and added to the totals), so a long run holds a bounded number of
+    // events.  Guarded by mu.
     bool timing = false;
-    std::vector<hipEvent_t> timing_events;
-    size_t timing_used = 0;
+    std::vector<hipEvent_t> timing_free;
+    std::deque<std::array<hipEvent_t, 3>> timing_pending;
+    dft2_kernel_timing timing_totals{};
 };
 
 namespace dft2 {

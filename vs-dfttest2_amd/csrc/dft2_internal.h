@@ -75,6 +75,9 @@ struct JobDev {
     const void* grad_out;
     void* grad_in;
     long long go_fp, go_rs, gi_fp, gi_rs;
+    // Gain-decision record (LaunchArgs::decisions): index of block (0, 0) of output frame
+    // f_begin of this job in the record.
+    long long dec_base;
 };
 
 struct LaunchArgs {
@@ -106,6 +109,11 @@ struct LaunchArgs {
     // fewer than 2^31 work items.
     unsigned* queue;
     int n_items;
+    // Gain-decision record of the hard-threshold filters (instantiations with REC only):
+    // per block (dec_base + (frame * nby + by) * nbx + bx), (radius+1)*4 64-bit masks
+    // [g][slot], bit `lane` = the decision of the coefficient middle-stage lane `lane`
+    // owns (filter type 1: psd < sigma, dropped; type 3: pmin <= psd <= pmax).
+    unsigned long long* decisions;
 };
 
 // Selects the fused kernel instantiation.
@@ -118,6 +126,9 @@ struct FusedKey {
     int asym;       // LaunchArgs::asym
     int sep;        // 1: separable-window form (constant table SWX/SWY/AS/CTS); the persistent
                     // form exists only with it
+    int rec;        // 1: records gain decisions (LaunchArgs::decisions); exists for filter
+                    // type 1 with a symmetric sigma and type 3 in FP32, one item per
+                    // workgroup
 };
 
 // Sample type codes.

@@ -25,6 +25,7 @@ EXPORTED_SYMBOLS = (
     "dft2_process_frame",
     "dft2_process_batch",
     "dft2_process_batch_sigma",
+    "dft2_process_batch_decisions",
     "dft2_backward_batch",
     "dft2_plan_set_timing",
     "dft2_plan_get_timing",
@@ -174,6 +175,10 @@ def load():
         ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
     ]
     lib.dft2_process_batch_sigma.restype = ctypes.c_int
+    lib.dft2_process_batch_decisions.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
+    ]
+    lib.dft2_process_batch_decisions.restype = ctypes.c_int
     lib.dft2_backward_batch.argtypes = [
         ctypes.c_void_p, ctypes.POINTER(Dft2GradJob), ctypes.c_int, ctypes.c_void_p, ctypes.c_float,
         ctypes.c_void_p, ctypes.c_void_p,
@@ -307,6 +312,14 @@ class Plan:
         arr = (Dft2Job * len(jobs))(*jobs)
         _check(load().dft2_process_batch_sigma(self._h, arr, len(jobs), ctypes.c_void_p(sigma_ptr),
                                                ctypes.c_float(sigma_scale), _stream_arg(stream)))
+
+    def process_batch_decisions(self, jobs: List[Dft2Job], decisions_ptr: int, decisions_len: int,
+                                stream: Optional[int] = None):
+        """As process_batch, also recording every block's hard gain decisions (filter types 1
+        and 3) into decisions_len 64-bit masks at device address decisions_ptr."""
+        arr = (Dft2Job * len(jobs))(*jobs)
+        _check(load().dft2_process_batch_decisions(self._h, arr, len(jobs), ctypes.c_void_p(decisions_ptr),
+                                                   ctypes.c_longlong(decisions_len), _stream_arg(stream)))
 
     def backward_batch(self, jobs: List[Dft2GradJob], sigma_ptr: int, sigma_scale: float, sigma_grad_ptr: int,
                        stream: Optional[int] = None):
