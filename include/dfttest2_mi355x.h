@@ -156,6 +156,18 @@ DFT2_API int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, 
  * `stream` (NULL = the plan's own stream, and the call waits for completion). */
 DFT2_API int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
 
+/* As dft2_process_batch, for callers that cannot allocate (code under stream capture,
+ * tight serving loops, harnesses that forbid hipMalloc): the call allocates nothing.  No
+ * reference counterpart.  Its working memory (job table and tile regions) is one 64 MiB
+ * array of device memory per GPU, held from library load until the process ends and
+ * shared by every call on that GPU: calls that overlap on one GPU (different streams or
+ * threads) race, so the caller serialises them.  At most 64 jobs per call; frames run in
+ * chunks that fit the array, and jobs whose single frame needs more tile scratch than it
+ * holds (about 3 frames of 1080p YUV420 fit) are rejected.  Kernel timing
+ * (dft2_plan_set_timing) does not record these launches.  NULL stream: the plan's first
+ * stream, and the call waits for completion. */
+DFT2_API int dft2_process_batch_noalloc(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
+
 /* As dft2_process_batch, with sigma read from device memory: sigma_dev holds T*16*9
  * floats [t][ky][kx] and the filter uses sigma_dev * sigma_scale (sigma_scale = the
  * window's sum of squares for ftype 0/1 in user units, as python/dfttest2.py:329-333).

@@ -27,6 +27,7 @@ SOURCES = tuple(("dft2_fused.hip", [f"-DDFT2_FUSED_RADIUS={r}"], f"dft2_fused_r{
     ("dft2_kernels.hip", [], "dft2_kernels.o"),
     ("dft2_capi.cpp", [], "dft2_capi.o"),
     ("dft2_batch.cpp", [], "dft2_batch.o"),
+    ("dft2_noalloc.cpp", [], "dft2_noalloc.o"),
 )
 
 # Packed FP32 instructions are kept out of every gfx950 code object.  The sign of a zero

@@ -26,6 +26,7 @@ EXPORTED_SYMBOLS = (
     "dft2_process_batch",
     "dft2_process_batch_sigma",
     "dft2_process_batch_decisions",
+    "dft2_process_batch_noalloc",
     "dft2_backward_batch",
     "dft2_plan_set_timing",
     "dft2_plan_get_timing",
@@ -179,6 +180,9 @@ def load():
         ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_void_p,
     ]
     lib.dft2_process_batch_decisions.restype = ctypes.c_int
+    lib.dft2_process_batch_noalloc.argtypes = [ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int,
+                                               ctypes.c_void_p]
+    lib.dft2_process_batch_noalloc.restype = ctypes.c_int
     lib.dft2_backward_batch.argtypes = [
         ctypes.c_void_p, ctypes.POINTER(Dft2GradJob), ctypes.c_int, ctypes.c_void_p, ctypes.c_float,
         ctypes.c_void_p, ctypes.c_void_p,
@@ -312,6 +316,11 @@ class Plan:
         arr = (Dft2Job * len(jobs))(*jobs)
         _check(load().dft2_process_batch_sigma(self._h, arr, len(jobs), ctypes.c_void_p(sigma_ptr),
                                                ctypes.c_float(sigma_scale), _stream_arg(stream)))
+
+    def process_batch_noalloc(self, jobs: List[Dft2Job], stream: Optional[int] = None):
+        """As process_batch, through the entry point that allocates nothing."""
+        arr = (Dft2Job * len(jobs))(*jobs)
+        _check(load().dft2_process_batch_noalloc(self._h, arr, len(jobs), _stream_arg(stream)))
 
     def process_batch_decisions(self, jobs: List[Dft2Job], decisions_ptr: int, decisions_len: int,
                                 stream: Optional[int] = None):
