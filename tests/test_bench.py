@@ -42,3 +42,24 @@ def test_roofline_peaks():
     seconds = bf16 / 2500e12 + (124_416 - bf16) / 157.3e12
     assert peak == pytest.approx(124_416 / seconds / 1e12, rel=1e-3)
     assert bench.PEAK_FP32_MFMA_TFLOPS < peak < bench.PEAK_BF16_MFMA_TFLOPS
+
+
+def test_mfma_counts_per_block():
+    # FP32, 8-bit: 3 BF16 16x16x16 per slice (exact split of the x basis), 4 FP32 16x16x4
+    # per y transform and per inverse contraction; split BF16: 2 and 3 x 16x16x16; 16-bit
+    # samples take the 16x16x32 for the x transform.
+    assert bench.mfma_per_block(3, "fp32", 1) == {"bf16_16x16x16": 9, "f32_16x16x4": 20}
+    assert bench.mfma_per_block(3, "split_bf16", 1) == {"bf16_16x16x16": 21}
+    assert bench.mfma_per_block(5, "fp32", 2) == {"bf16_16x16x32": 15, "f32_16x16x4": 28}
+    assert bench.mfma_per_block(5, "split_bf16", 2) == {"bf16_16x16x32": 10, "bf16_16x16x16": 21}
+    e = bench.executed({"f32_16x16x4": 20, "bf16_16x16x16": 9}, 1e6, 1.0)
+    assert e["flop_per_block"] == 20 * 2048 + 9 * 8192
+    # 1e6 blocks x 784 cycles in 1 ms on 1024 SIMDs at 2.4 GHz
+    assert e["mfma_pipe_busy"] == pytest.approx(1e6 * 784 / (1e-3 * 1024 * 2.4e9), rel=1e-3)
+
+
+def test_cpu_share_respects_omp_threads(monkeypatch):
+    monkeypatch.setenv("OMP_NUM_THREADS", "3")
+    assert bench.cpu_threads() == min(3, len(os.sched_getaffinity(0)))
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.cpu_threads() == len(os.sched_getaffinity(0))

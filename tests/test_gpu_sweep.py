@@ -24,7 +24,7 @@ import pytest
 from dfttest2_amd import _lib, synth
 from dfttest2_amd.dfttest2 import Backend, VideoClip, filter_clip, get_plan
 from dfttest2_amd.params import dfttest_params
-from decision_util import run_with_decisions
+from dfttest2_amd.decisions import run_with_decisions
 from oracle import dfttest_np as onp
 
 pytestmark = pytest.mark.gpu
@@ -66,8 +66,7 @@ def bar_of(prec, bits, is_float, p):
 def check_hard_decisions(x, p, bits, is_float, prec, bar, label):
     """Decision-replay check of one clip (filter types 1 and 3); returns the output."""
     plan = get_plan(p, "float" if is_float else "int", bits, Backend.MI355X(precision=prec))
-    _, _, ny, nx, _, _ = onp._geometry(x, p)
-    out, dec_gpu = run_with_decisions(plan, x, p, (ny, nx))
+    out, dec_gpu = run_with_decisions(plan, x, p)
     plain = filter_clip(VideoClip([x], "float" if is_float else "int", bits), p,
                         Backend.MI355X(precision=prec)).planes[0]
     np.testing.assert_array_equal(out, plain)  # recording leaves the arithmetic unchanged

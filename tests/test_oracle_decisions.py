@@ -150,7 +150,7 @@ def test_margins_cover_emulated_arithmetic(form):
 def test_decision_record_layout_round_trip(t_len):
     # Every coefficient is decided by some lane (directly, or for kx = 0, 8 and kt > r by
     # its conjugate), coefficients held twice agree, and decoding inverts the layout.
-    from decision_util import decode, lane_positions
+    from dfttest2_amd.decisions import decode, lane_positions
 
     r = t_len // 2
     rng = np.random.default_rng(t_len)

@@ -1,5 +1,8 @@
-"""Runs the hard-threshold filters with decision records (dft2_process_batch_decisions) and
-decodes the records into the per-coefficient layout the oracle replays.
+"""Decision records of the hard-threshold filters (dft2_process_batch_decisions): runs a
+clip with them and decodes them into per-coefficient decisions [..., T, 16, 9], the
+layout of the reference's spectrum (t, ky, kx) (nvrtc_source/kernels.cu:45).  Parity
+instrumentation: a checker replays them to show where another arithmetic's hard
+decisions differ.
 
 Record layout (include/dfttest2_mi355x.h): per output frame and block, (radius+1)*4
 64-bit masks [g][slot]; bit `lane` is the decision on the coefficient middle-stage lane
@@ -11,7 +14,7 @@ decisions.
 
 import numpy as np
 
-from dfttest2_amd.dfttest2 import plane_job
+from .dfttest2 import plane_job
 
 
 def lane_positions(t_len):
@@ -57,13 +60,19 @@ def decode(masks, t_len):
     return dec
 
 
-def run_with_decisions(plan, x, p, block_counts):
+def _pad_num(size: int, block_step: int) -> int:
+    """Blocks along one axis (cpu_source/getframe_impl.cpp:11-20)."""
+    ps = size + ((16 - size % 16) if size % 16 else 0) + max(16 - block_step, block_step) * 2
+    return (ps - 16) // block_step + 1
+
+
+def run_with_decisions(plan, x, p):
     """Filters the clip x [N, H, W] (numpy) on the GPU through the decision-recording entry;
     returns (output, decisions per output frame [ny, nx, T, 16, 9])."""
     import torch
 
-    n = x.shape[0]
-    ny, nx = block_counts
+    n, h, w = x.shape
+    ny, nx = _pad_num(h, p.block_step), _pad_num(w, p.block_step)
     t_len = p.t_len
     src = torch.from_numpy(np.ascontiguousarray(x)).cuda()
     dst = torch.empty_like(src)
