@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
     const int lg = lane >> 4;
 
     const int item = blockIdx.x;
-    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const JobDev& job = a.jobs[sgpr(find_job<0>(a.jobs, a.n_jobs, item))];
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles = tiles_x * sgpr(job.tiles_y);
     const int local = item - sgpr(static_cast<int>(job.item_begin));

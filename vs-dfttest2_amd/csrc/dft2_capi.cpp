@@ -97,6 +97,63 @@ TileShape make_tile(const dft2_plan* p, int bw, int bh) {
     return ts;
 }
 
+// Cross-frame form of plan p: frames per work item and tile.  It exists for separable
+// windows at radius >= 1.  Default: 8 frames per item from radius 2 (tbsize 5 and 7, where
+// the spatial transforms of T slices take most MFMA cycles; 4K 10-bit tbsize 5, FP32:
+// 23.2 -> 21.8 ms per 24 frames at 4, 20.4 at 8, the most rings for which two workgroups
+// still fit a CU); radius 1 keeps the one-frame form (its extra live registers cost a
+// third of the resident waves: 1080p 8-bit tbsize 3, 33.2 -> 41.3 ms at 4 frames).
+// DFT2_REUSE=K and DFT2_REUSE_TILE=WxH override.  The tile: 8 blocks wide (one per wave
+// and pass), and the tallest candidate among those with the most resident workgroups:
+// its accumulators are rings of 16-row strips, so height costs no LDS, and taller tiles
+// have fewer overlap strips to write and gather.
+void choose_reuse(dft2_plan* p) {
+    p->reuse_k = 1;
+    if (!p->separable || p->radius < 1) {
+        return;
+    }
+    int k = p->radius >= 2 ? 8 : 1;
+    if (const char* env = std::getenv("DFT2_REUSE")) {
+        k = std::max(1, std::min(8, std::atoi(env)));
+    }
+    if (k <= 1) {
+        return;
+    }
+    const int tb_max = 84 / p->block_step + 1;
+    int bw = std::min(8, tb_max);
+    int best_wg = -1;
+    TileShape best;
+    int forced_w = 0, forced_h = 0;
+    if (const char* env = std::getenv("DFT2_REUSE_TILE")) {
+        std::sscanf(env, "%dx%d", &forced_w, &forced_h);
+    }
+    static const int kHeights[] = {32, 24, 16, 12, 8};
+    for (int bh0 : kHeights) {
+        const int w = forced_w > 0 ? forced_w : bw;
+        const int h = forced_h > 0 ? forced_h : std::min(bh0, 64);
+        if (!tile_valid(p, w, h)) {
+            continue;
+        }
+        TileShape ts = make_tile(p, w, h);
+        ts.lds = fused_reuse_lds_bytes(p->radius, ts.region_w, p->block_step, k);
+        FusedKey key = fused_key(p);
+        key.reuse = 1;
+        const int wg = fused_workgroups_per_cu(key, ts.lds);
+        if (wg > best_wg) {
+            best_wg = wg;
+            best = ts;
+        }
+        if (forced_h > 0) {
+            break;
+        }
+    }
+    if (best_wg <= 0) {
+        return;
+    }
+    p->reuse_k = k;
+    p->reuse_tile = best;
+}
+
 } // namespace host
 } // namespace dft2
 
@@ -366,6 +423,8 @@ LaunchArgs make_args(const dft2_plan* p, const TileShape& ts) {
     a.queue = nullptr;
     a.n_items = 0;
     a.decisions = nullptr;
+    a.reuse_k = 1;
+    a.ring_bands = (16 + p->block_step - 1) / p->block_step;
     return a;
 }
 
@@ -414,7 +473,7 @@ int upload_jobs(Slot* slot, const JobDev* src_table, size_t table_bytes, hipStre
 }
 
 int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDev>& jobs_in, size_t scratch_cap,
-                ChunkPlan& out) {
+                ChunkPlan& out, int reuse_k) {
     const int n_jobs = static_cast<int>(jobs_in.size());
     const long long region = static_cast<long long>(ts.region_w) * ts.region_h;
     long long per_frame_items = 0;
@@ -450,8 +509,10 @@ int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDe
             j.dec_base += static_cast<long long>(f0) * j.nby * j.nbx;
             j.item_begin = ch.items;
             j.frame_begin = ch.frames;
+            j.group_begin = ch.groups;
             ch.items += static_cast<long long>(cnt) * j.tiles_x * j.tiles_y;
             ch.frames += cnt;
+            ch.groups += static_cast<long long>((cnt + reuse_k - 1) / reuse_k) * j.tiles_x * j.tiles_y;
             if (cnt > 0) {
                 ch.max_pixels = std::max(ch.max_pixels, j.width * j.height);
             }
@@ -536,9 +597,22 @@ int build_gain_table(const dft2_plan* p, Slot* slot, LaunchArgs& a, bool bias, h
 int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
              const float* sigma_dev = nullptr, float sigma_scale = 1.0f,
              unsigned long long* decisions = nullptr) {
-    const TileShape& ts = p->tile;
+    // The cross-frame form when the plan takes it (it records no decisions).
+    const int reuse_k = decisions ? 1 : p->reuse_k;
+    const TileShape& ts = reuse_k > 1 ? p->reuse_tile : p->tile;
+    // Job geometry counts tiles of p->tile; recount them for the form's tile.
+    std::vector<JobDev> jobs_t;
+    const std::vector<JobDev>* jobs_use = &jobs_in;
+    if (reuse_k > 1) {
+        jobs_t = jobs_in;
+        for (auto& j : jobs_t) {
+            j.tiles_x = (j.nbx + ts.bw - 1) / ts.bw;
+            j.tiles_y = (j.nby + ts.bh - 1) / ts.bh;
+        }
+        jobs_use = &jobs_t;
+    }
     ChunkPlan cp;
-    int rc = plan_chunks(p, ts, jobs_in, p->scratch_cap, cp);
+    int rc = plan_chunks(p, ts, *jobs_use, p->scratch_cap, cp, reuse_k);
     if (rc || cp.chunks.empty()) {
         return rc;
     }
@@ -570,6 +644,10 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         a.decisions = decisions;
         key.rec = 1;
     }
+    if (reuse_k > 1) {
+        a.reuse_k = reuse_k;
+        key.reuse = 1;
+    }
     for (size_t c = 0; c < cp.chunks.size(); c++) {
         const Chunk& ch = cp.chunks[c];
         if (ch.items == 0) {
@@ -584,7 +662,7 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         if (ev[0]) {
             HIP_TRY(hipEventRecord(ev[0], stream));
         }
-        HIP_TRY(launch_fused(a, key, static_cast<unsigned>(ch.items), ts.lds, stream));
+        HIP_TRY(launch_fused(a, key, static_cast<unsigned>(reuse_k > 1 ? ch.groups : ch.items), ts.lds, stream));
         if (ev[1]) {
             HIP_TRY(hipEventRecord(ev[1], stream));
         }
@@ -773,6 +851,7 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
         return rc;
     }
     p->tile = choose_tile(p);
+    choose_reuse(p);
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_consts), sizeof(float) * consts.size());
     if (e == hipSuccess) {
         e = hipMemcpy(p->d_consts, consts.data(), sizeof(float) * consts.size(), hipMemcpyHostToDevice);

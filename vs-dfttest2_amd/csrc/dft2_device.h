@@ -522,15 +522,16 @@ __device__ __forceinline__ int sigma_index(int3 p) {
     return (p.x * 16 + p.y) * 9 + p.z;
 }
 
-// Index of the last job whose first work item (or first job-frame) is <= v: the job
-// table is sorted by both.  Jobs without work in a chunk share their start with the
+// Index of the last job whose first work item (kKey 0), first job-frame (1) or first
+// frame-group work item (2, the cross-frame form of the fused kernel) is <= v: the job
+// table is sorted by each.  Jobs without work in a chunk share their start with the
 // next job and are skipped by taking the last match.
-template <bool kByFrame>
+template <int kKey>
 __device__ __forceinline__ int find_job(const JobDev* jobs, int n_jobs, long long v) {
     int lo = 0, hi = n_jobs - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        const long long start = kByFrame ? jobs[mid].frame_begin : jobs[mid].item_begin;
+        const long long start = kKey == 2 ? jobs[mid].group_begin : (kKey == 1 ? jobs[mid].frame_begin : jobs[mid].item_begin);
         if (start <= v) {
             lo = mid;
         } else {

@@ -45,13 +45,27 @@ namespace dft2 {
 namespace {
 
 // Radius 0/1, Wiener or threshold gain, separable window: at most 80 VGPRs, so that six
-// waves per SIMD (three workgroups per CU) fit.
+// waves per SIMD (three workgroups per CU) fit.  The cross-frame form keeps T slice
+// spectra live across its frames: at most 128 VGPRs at radius 1 and 2 (two workgroups
+// per CU, which its LDS also allows).
 // REC: also writes every block's gain decisions to a.decisions (parity instrumentation of
 // the hard-threshold filters; separate instantiations, so the others carry no trace of it).
-template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP, bool REC = false>
+// REUSE: the cross-frame form.  A work item is a tile over K = a.reuse_k consecutive
+// output frames; each wave keeps its block's T slice spectra D2 in registers and slides
+// them from one frame to the next, so each further frame computes the spatial transform
+// of one new slice instead of T (the reference recomputes every slice for every output
+// frame, cpu_source/getframe_impl.cpp:330-386).  The separable window makes a slice's D2
+// independent of its position in the window (the slice factors sit in the temporal
+// basis).  Each frame's accumulator is a ring of ceil(16/S) bands of S rows: band r is
+// final once block row r is done, and is then copied to the frame's scratch region and
+// cleared, so K accumulators take the LDS of about one 16-row strip each.
+template <int R, typename Tin, int FTM, int PREC, bool PERSIST, int ASYM, bool SEP, bool REC = false,
+          bool REUSE = false>
 __global__ __launch_bounds__(kThreads)
-__attribute__((amdgpu_waves_per_eu(R <= 1 && FTM != 2 && SEP && !PERSIST && !REC ? 6 : 1)))
+__attribute__((amdgpu_waves_per_eu(REUSE ? (R <= 2 ? 4 : 1)
+                                          : (R <= 1 && FTM != 2 && SEP && !PERSIST && !REC ? 6 : 1))))
 void fused_kernel(LaunchArgs a) {
+    static_assert(!REUSE || (SEP && !PERSIST && !REC && R >= 1), "cross-frame form: separable, one item, R >= 1");
     constexpr int T = 2 * R + 1;
     // Overlap-add: staged block outputs summed by the pixels' owners after a barrier, in a
     // fixed order (LDS float atomics, the first form, cost more at every radius: 4K tbsize
@@ -87,8 +101,12 @@ void fused_kernel(LaunchArgs a) {
 
     const int S = a.block_step;
     const int rw = a.region_w;
+    // Accumulators: the full tile region, or (REUSE) K rings of nbands bands of S rows.
+    const int K = REUSE ? sgpr(a.reuse_k) : 1;
+    const int nbands = REUSE ? sgpr(a.ring_bands) : 0;
+    const int ring_floats = nbands * S * rw;
     float* acc = smem;
-    float4* tab = reinterpret_cast<float4*>(smem + a.region_h * rw);
+    float4* tab = reinterpret_cast<float4*>(smem + (REUSE ? K * ring_floats : a.region_h * rw));
     float* stage = reinterpret_cast<float*>(tab + (R + 1) * 4 * 64);
     float* wbuf = stage + wave * kWbuf;
     fill_tab<R, FTM == 0>(tab, a, tid, kThreads);
@@ -166,13 +184,16 @@ void fused_kernel(LaunchArgs a) {
     const int wc = l16 >> 3;
 
     while (item < a.n_items) {
-    // ---- work item -> (job, frame, tile) ----
-    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    // ---- work item -> (job, frame, tile); REUSE: (job, first frame of a group of K, tile) ----
+    const JobDev& job = a.jobs[sgpr(find_job<REUSE ? 2 : 0>(a.jobs, a.n_jobs, item))];
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles = tiles_x * sgpr(job.tiles_y);
-    const int local = item - sgpr(static_cast<int>(job.item_begin));
-    const int f_local = local / tiles;
-    const int tile = local - f_local * tiles;
+    const int local = item - sgpr(static_cast<int>(REUSE ? job.group_begin : job.item_begin));
+    const int group = local / tiles;
+    const int tile = local - group * tiles;
+    const int f_local = group * K;
+    // frames of this item (K, fewer at the end of the job's frame range)
+    const int kcount = REUSE ? min(K, sgpr(job.f_count) - f_local) : 1;
     const int ty = tile / tiles_x;
     const int tx = tile - ty * tiles_x;
     const int n = sgpr(job.f_begin) + f_local;
@@ -190,31 +211,38 @@ void fused_kernel(LaunchArgs a) {
     const int rs = sgpr(static_cast<int>(job.src_rs));
     const unsigned lane_off = static_cast<unsigned>(l16 * rs + 4 * lg);
 
-    // The region (rows_used x rw, rw a multiple of 4) is zeroed and read out as float4.
+    // The region (rows_used x rw, rw a multiple of 4), or the K rings, zeroed; read out as
+    // float4.
     float4* acc4 = reinterpret_cast<float4*>(acc);
-    for (int i = tid; i < rows_used * rw / 4; i += kThreads) {
+    for (int i = tid; i < (REUSE ? K * ring_floats : rows_used * rw) / 4; i += kThreads) {
         acc4[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
 
     // Source planes of the T slices (temporal clamp, cpu_source/getframe_impl.cpp:294-298).
     // wide_ok: every plane base and the row pitch are multiples of kRaw4Align, so blocks
     // whose first sample is too take one wide load per slice.
+    // REUSE: frame n + k (k >= 1) adds the source slice n + k + R, whose descriptor is made
+    // when its samples are fetched (slice_rsrc).
     __amdgpu_buffer_rsrc_t frame[T];
     constexpr unsigned kAl = kRaw4Align<Tin>;
     bool wide_ok = (static_cast<unsigned>(rs) * sizeof(Tin)) % kAl == 0;
-    {
-        const int n_frames = sgpr(job.n_frames);
-        const bool use_fptr = sgpr(job.use_fptr) != 0;
-        const Tin* src = sgpr_ptr(reinterpret_cast<const Tin*>(job.src));
-        const long long fp = sgpr64(job.src_fp);
-        const unsigned frame_bytes = static_cast<unsigned>(rs * height) * static_cast<unsigned>(sizeof(Tin));
+    const int n_frames = sgpr(job.n_frames);
+    const unsigned frame_bytes = static_cast<unsigned>(rs * height) * static_cast<unsigned>(sizeof(Tin));
+    auto slice_base = [&](int fi) -> const Tin* {
+        fi = min(max(fi, 0), n_frames - 1);
+        return sgpr(job.use_fptr) != 0 ? sgpr_ptr(reinterpret_cast<const Tin*>(job.fptr[fi]))
+                                       : sgpr_ptr(reinterpret_cast<const Tin*>(job.src)) + static_cast<long long>(fi) *
+                                                                                         sgpr64(job.src_fp);
+    };
 #pragma unroll
-        for (int t = 0; t < T; t++) {
-            const int fi = min(max(n - R + t, 0), n_frames - 1);
-            const Tin* base = use_fptr ? sgpr_ptr(reinterpret_cast<const Tin*>(job.fptr[fi]))
-                                       : src + static_cast<long long>(fi) * fp;
-            frame[t] = frame_rsrc(base, frame_bytes);
-            wide_ok = wide_ok && (reinterpret_cast<unsigned long long>(base) % kAl) == 0;
+    for (int t = 0; t < T; t++) {
+        const Tin* base = slice_base(n - R + t);
+        frame[t] = frame_rsrc(base, frame_bytes);
+        wide_ok = wide_ok && (reinterpret_cast<unsigned long long>(base) % kAl) == 0;
+    }
+    if constexpr (REUSE) {
+        for (int k = 1; k < kcount; k++) {
+            wide_ok = wide_ok && (reinterpret_cast<unsigned long long>(slice_base(n + k + R)) % kAl) == 0;
         }
     }
     // Samples of block (fy, fx) of the tile for this lane: row y = lane%16, columns
@@ -222,7 +250,9 @@ void fused_kernel(LaunchArgs a) {
     // (most of them) take one offset per lane, and one wide load per slice when aligned;
     // the others apply the mirror padding per sample.
     constexpr unsigned kSz = sizeof(Tin);
-    auto fetch = [&](int fy, int fx, raw4<Tin> (&dst)[T]) {
+    // fetch(fy, fx, fr, dst): slices from the N descriptors fr[] into dst[0..N-1].
+    auto fetch = [&](int fy, int fx, const auto& fr, auto& dst) {
+        constexpr int N = sizeof(fr) / sizeof(fr[0]);
         fx = opaque_uniform(fx); // a wave's column is fixed: keep its offsets out of the loop's registers
         const int y0 = (by0 + fy) * S - oy;
         const int x0 = (bx0 + fx) * S - ox;
@@ -230,14 +260,14 @@ void fused_kernel(LaunchArgs a) {
             const unsigned off = (static_cast<unsigned>(y0 * rs + x0) + lane_off) * kSz;
             if (wide_ok && (static_cast<unsigned>(y0 * rs + x0) * kSz) % kAl == 0) {
 #pragma unroll
-                for (int t = 0; t < T; t++) {
-                    dst[t] = load4_aligned<Tin>(frame[t], off);
+                for (int t = 0; t < N; t++) {
+                    dst[t] = load4_aligned<Tin>(fr[t], off);
                 }
             } else {
                 const unsigned offs[4] = {off, off + kSz, off + 2 * kSz, off + 3 * kSz};
 #pragma unroll
-                for (int t = 0; t < T; t++) {
-                    dst[t] = load4_each<Tin>(frame[t], offs);
+                for (int t = 0; t < N; t++) {
+                    dst[t] = load4_each<Tin>(fr[t], offs);
                 }
             }
         } else {
@@ -248,8 +278,26 @@ void fused_kernel(LaunchArgs a) {
                 offs[s] = (roff + static_cast<unsigned>(mirror(x0 + 4 * lg + s, width))) * kSz;
             }
 #pragma unroll
-            for (int t = 0; t < T; t++) {
-                dst[t] = load4_each<Tin>(frame[t], offs);
+            for (int t = 0; t < N; t++) {
+                dst[t] = load4_each<Tin>(fr[t], offs);
+            }
+        }
+    };
+
+    // REUSE: copies band b (region rows b*S .. b*S+S-1, fewer past rows_used) of every frame
+    // of the item from ring slot `slot` to the frame's scratch region and zeroes the slot.
+    // Scratch region of frame f_local + k = the work item (f_local + k, tile) of the
+    // one-frame form, which finalize reads.
+    auto flush_band = [&](int b, int slot) {
+        const int rows = min(S, rows_used - b * S);
+        const int n4 = rows * rw / 4;
+        for (int k = 0; k < kcount; k++) {
+            float4* src = reinterpret_cast<float4*>(acc + k * ring_floats + slot * S * rw);
+            const long long region_item = sgpr(static_cast<int>(job.item_begin)) + static_cast<long long>(f_local + k) * tiles + tile;
+            float4* dst = reinterpret_cast<float4*>(a.scratch + region_item * (a.region_h * rw) + b * S * rw);
+            for (int i = tid; i < n4; i += kThreads) {
+                dst[i] = src[i];
+                src[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             }
         }
     };
@@ -261,28 +309,43 @@ void fused_kernel(LaunchArgs a) {
     const int my_pass = nbx > wave ? (nbx - wave + kWaves - 1) / kWaves : 0; // passes with a block of this wave
     raw4<Tin> nxt[T];
     if (my_pass > 0) {
-        fetch(0, wave, nxt);
+        fetch(0, wave, frame, nxt);
     }
 
     __syncthreads();
 
+    floatx4 d2[T]; // slice spectra; REUSE slides them from one frame of the item to the next
+    int r_band = 0; // REUSE: r mod nbands, the ring slot of band r
     for (int r = 0; r < nby; r++) {
     for (int pass = 0; pass < n_pass; pass++) {
+    for (int k = 0; k < kcount; k++) {
     if (pass < my_pass) {
+        // Slices to transform: all T for the item's first frame, else the one new slice,
+        // which the prefetch put in nxt[T - 1].
+        const bool all = !REUSE || k == 0;
         raw4<Tin> cur_raw[T]; // the samples as loaded (16-bit samples' exact path)
-        float cur[T][4];
+        float cur[T][4];      // and as floats (REUSE: converted where used, one slice or T)
 #pragma unroll
         for (int t = 0; t < T; t++) {
             cur_raw[t] = nxt[t];
+            if constexpr (!REUSE) {
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                cur[t][s] = raw4_value<Tin>(nxt[t], s);
+                for (int s = 0; s < 4; s++) {
+                    cur[t][s] = raw4_value<Tin>(nxt[t], s);
+                }
             }
         }
         {
-            const bool row_end = pass + 1 == my_pass;
-            if (!row_end || r + 1 < nby) {
-                fetch(row_end ? r + 1 : r, (row_end ? 0 : pass + 1) * kWaves + wave, nxt);
+            if (REUSE && k + 1 < kcount) {
+                // the same block at the next frame: only its newest slice
+                const __amdgpu_buffer_rsrc_t fr1[1] = {frame_rsrc(slice_base(n + k + 1 + R), frame_bytes)};
+                raw4<Tin>(&last)[1] = *reinterpret_cast<raw4<Tin>(*)[1]>(&nxt[T - 1]);
+                fetch(r, pass * kWaves + wave, fr1, last);
+            } else {
+                const bool row_end = pass + 1 == my_pass;
+                if (!row_end || r + 1 < nby) {
+                    fetch(row_end ? r + 1 : r, (row_end ? 0 : pass + 1) * kWaves + wave, frame, nxt);
+                }
             }
         }
 
@@ -294,9 +357,23 @@ void fused_kernel(LaunchArgs a) {
         // enter as their two bytes on the 16x16x32 BF16 MFMA (mm16_bytes).
         constexpr bool kExactIn = SEP && sizeof(Tin) == 1;
         constexpr bool kExactIn16 = SEP && sizeof(Tin) == 2;
-        floatx4 d2[T];
+        if (!all) {
+#pragma unroll
+            for (int t = 0; t + 1 < T; t++) {
+                d2[t] = d2[t + 1];
+            }
+        }
 #pragma unroll
         for (int t = 0; t < T; t++) {
+            if (!all && t + 1 < T) {
+                continue;
+            }
+            if constexpr (REUSE) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    cur[t][s] = raw4_value<Tin>(cur_raw[t], s);
+                }
+            }
             const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
             floatx4 d1;
             if constexpr (kExactIn && PREC == 1) {
@@ -468,7 +545,7 @@ void fused_kernel(LaunchArgs a) {
             srow[i * kOut] = SEP ? o[i] : o[i] * syn[i];
         }
     }
-        // ---- overlap-add of pass `pass` of block row r, without atomics ----
+        // ---- overlap-add of pass `pass` of block row r (REUSE: of frame k), without atomics ----
         // Every pixel (y, x) of the pass's 16-row band sums the staged outputs of the
         // pass's blocks covering it; the thread that owns the pixel in this phase adds the
         // sum to the accumulator.  With x local to the pass (origin: its first block), the
@@ -490,31 +567,62 @@ void fused_kernel(LaunchArgs a) {
                 // bounds and masks serve the wave's rows y = wave, wave + 8
                 const int lo = div_small(max(x + Sg - 16, 0), Sm);
                 const int hi = min(div_small(x, Sm), nb - 1);
-                float* acol = acc + (r * Sg + wave) * rw + pass * kWaves * Sg + x;
                 // All reads of the wave's rows (accumulator and staged values) first, one
                 // LDS round trip, then the sums and the writes.
                 constexpr int kRows = 16 / kWaves;
+                // Accumulator row of the wave's row i (r * S + dy, dy = i * kWaves + wave):
+                // at (r * S + dy) * rw, or (REUSE) in frame k's ring, band r + dy / S in slot
+                // (r + dy / S) mod nbands.
+                float* acol = acc + (r * Sg + wave) * rw + pass * kWaves * Sg + x;
+                float* arow[kRows];
+#pragma unroll
+                for (int i = 0; i < kRows; i++) {
+                    if constexpr (REUSE) {
+                        const int dy = i * kWaves + wave;
+                        const int q = div_small(dy, Sm);
+                        int band = r_band + q;
+                        band = band >= nbands ? band - nbands : band;
+                        arow[i] = acc + k * ring_floats + (band * Sg + dy - q * Sg) * rw + pass * kWaves * Sg + x;
+                    } else {
+                        arow[i] = acol + i * kWaves * rw;
+                    }
+                }
                 const int k0 = max(hi - 3, 0);
                 float old[kRows], sum[kRows];
 #pragma unroll
                 for (int i = 0; i < kRows; i++) {
-                    old[i] = acol[i * kWaves * rw];
+                    old[i] = *arow[i];
                     sum[i] = gather_head<kWbuf>(stage + (i * kWaves + wave) * kOut + x, k0, lo, hi);
                 }
 #pragma unroll
                 for (int i = 0; i < kRows; i++) {
                     sum[i] = gather_tail<kWbuf>(stage + (i * kWaves + wave) * kOut + x, k0, lo, sum[i]);
-                    acol[i * kWaves * rw] = old[i] + sum[i];
+                    *arow[i] = old[i] + sum[i];
                 }
             }
         }
         __syncthreads(); // staged outputs consumed before the next pass reuses the buffers
+    } // k
+    } // pass
+    if constexpr (REUSE) {
+        // Band r is final in every frame of the item (later block rows start below it):
+        // copy it to the frames' scratch regions and clear its ring slot for band r +
+        // nbands.  The next writes to the slot follow the next pass's first barrier.
+        flush_band(r, r_band);
+        r_band = r_band + 1 == nbands ? 0 : r_band + 1;
     }
-    }
+    } // r
 
-    float4* out = reinterpret_cast<float4*>(a.scratch + static_cast<long long>(item) * (a.region_h * rw));
-    for (int i = tid; i < rows_used * rw / 4; i += kThreads) {
-        out[i] = acc4[i];
+    if constexpr (REUSE) {
+        for (int b = nby; b * S < rows_used; b++) {
+            flush_band(b, r_band);
+            r_band = r_band + 1 == nbands ? 0 : r_band + 1;
+        }
+    } else {
+        float4* out = reinterpret_cast<float4*>(a.scratch + static_cast<long long>(item) * (a.region_h * rw));
+        for (int i = tid; i < rows_used * rw / 4; i += kThreads) {
+            out[i] = acc4[i];
+        }
     }
     if constexpr (!PERSIST) {
         break;
@@ -529,10 +637,25 @@ void fused_kernel(LaunchArgs a) {
 }
 
 template <int R, typename Tin, int PREC, bool PERSIST, bool SEP>
-const void* fused_ptr_ft(int filter_type, bool asym, bool rec) {
+const void* fused_ptr_ft(int filter_type, bool asym, bool rec, bool reuse) {
     // Gains: 0 Wiener, 1 hard threshold (each with a symmetric and an asymmetric form),
     // 2 the generic form for types 2..6.  Decision records exist for type 1 with a
-    // symmetric sigma and for type 3 in FP32, one item per workgroup.
+    // symmetric sigma and for type 3 in FP32, one item per workgroup.  The cross-frame
+    // form exists for separable windows at radius >= 1, one item per workgroup.
+    if (reuse) {
+        if constexpr (!PERSIST && SEP && R >= 1) {
+            if (filter_type == 0) {
+                return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, false, 1, true, false, true>)
+                            : reinterpret_cast<const void*>(fused_kernel<R, Tin, 0, PREC, false, 0, true, false, true>);
+            }
+            if (filter_type == 1) {
+                return asym ? reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, false, 1, true, false, true>)
+                            : reinterpret_cast<const void*>(fused_kernel<R, Tin, 1, PREC, false, 0, true, false, true>);
+            }
+            return reinterpret_cast<const void*>(fused_kernel<R, Tin, 2, PREC, false, 2, true, false, true>);
+        }
+        return nullptr;
+    }
     if (rec) {
         if constexpr (!PERSIST) {
             if (filter_type == 1 && !asym) {
@@ -565,15 +688,15 @@ const void* fused_ptr_prec(const FusedKey& k) {
         if (!k.sep) {
             return nullptr;
         }
-        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, true, true>(k.filter_type, k.asym != 0, k.rec != 0)
-                                : fused_ptr_ft<R, Tin, 0, true, true>(k.filter_type, k.asym != 0, k.rec != 0);
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, true, true>(k.filter_type, k.asym != 0, k.rec != 0, k.reuse != 0)
+                                : fused_ptr_ft<R, Tin, 0, true, true>(k.filter_type, k.asym != 0, k.rec != 0, k.reuse != 0);
     } else {
         if (k.sep) {
-            return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0, k.rec != 0)
-                                    : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0, k.rec != 0);
+            return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, true>(k.filter_type, k.asym != 0, k.rec != 0, k.reuse != 0)
+                                    : fused_ptr_ft<R, Tin, 0, false, true>(k.filter_type, k.asym != 0, k.rec != 0, k.reuse != 0);
         }
-        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, false>(k.filter_type, k.asym != 0, k.rec != 0)
-                                : fused_ptr_ft<R, Tin, 0, false, false>(k.filter_type, k.asym != 0, k.rec != 0);
+        return k.precision == 1 ? fused_ptr_ft<R, Tin, 1, false, false>(k.filter_type, k.asym != 0, k.rec != 0, k.reuse != 0)
+                                : fused_ptr_ft<R, Tin, 0, false, false>(k.filter_type, k.asym != 0, k.rec != 0, k.reuse != 0);
     }
 }
 

@@ -79,6 +79,7 @@ struct TileShape {
 struct Chunk {
     long long items = 0;  // work items (tiles x frames)
     long long frames = 0; // job-frames (finalize grid y)
+    long long groups = 0; // cross-frame form: work items (tiles x groups of reuse_k frames)
     int max_pixels = 0;   // largest plane among jobs with work (finalize grid x)
 };
 
@@ -110,6 +111,10 @@ struct dft2_plan {
     int peak = 255;
     float sigma2 = 0, pmin = 0, pmax = 0, wf0 = 1.0f;
     dft2::host::TileShape tile; // default forward tile
+    // Cross-frame form (process_batch and frame calls without decision records): output
+    // frames per work item (1 = the one-frame form) and its tile.
+    int reuse_k = 1;
+    dft2::host::TileShape reuse_tile;
     size_t scratch_cap = size_t(1) << 30;
     float* d_consts = nullptr;
     std::vector<dft2::host::Slot> slots;
@@ -143,6 +148,8 @@ bool tile_valid(const dft2_plan* p, int bw, int bh);
 // Default forward tile: the candidate with the most blocks in flight per CU (resident
 // workgroups from the occupancy API x blocks per tile); needs the plan's device current.
 TileShape choose_tile(const dft2_plan* p);
+// Sets the plan's cross-frame form (reuse_k, reuse_tile); needs the device current.
+void choose_reuse(dft2_plan* p);
 
 // Launch arguments common to every kernel of a plan, for tile `ts`.
 LaunchArgs make_args(const dft2_plan* p, const TileShape& ts);
@@ -151,8 +158,9 @@ LaunchArgs make_args(const dft2_plan* p, const TileShape& ts);
 int make_job(const dft2_plan* p, const dft2_job& uj, const TileShape& ts, JobDev& j);
 
 // Splits jobs into frame chunks whose scratch fits `scratch_cap` bytes.
+// reuse_k > 1: also numbers the cross-frame form's work items (groups of reuse_k frames).
 int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDev>& jobs, size_t scratch_cap,
-                ChunkPlan& out);
+                ChunkPlan& out, int reuse_k = 1);
 
 int ensure_device(void** ptr, size_t* cap, size_t bytes);
 

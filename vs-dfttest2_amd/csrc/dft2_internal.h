@@ -78,6 +78,9 @@ struct JobDev {
     // Gain-decision record (LaunchArgs::decisions): index of block (0, 0) of output frame
     // f_begin of this job in the record.
     long long dec_base;
+    // Cross-frame form (LaunchArgs::reuse_k > 1): first work item of this job, one per
+    // (group of reuse_k consecutive output frames, tile).
+    long long group_begin;
 };
 
 struct LaunchArgs {
@@ -114,6 +117,12 @@ struct LaunchArgs {
     // [g][slot], bit `lane` = the decision of the coefficient middle-stage lane `lane`
     // owns (filter type 1: psd < sigma, dropped; type 3: pmin <= psd <= pmax).
     unsigned long long* decisions;
+    // Cross-frame form of the fused kernel: a work item is a tile over reuse_k consecutive
+    // output frames, which share the spatial spectra of their common source slices; each
+    // frame's accumulator is a ring of ring_bands bands of block_step rows (ring_bands =
+    // ceil(16 / block_step)).  reuse_k = 1: one frame per work item, full-region accumulator.
+    int reuse_k;
+    int ring_bands;
 };
 
 // Selects the fused kernel instantiation.
@@ -129,6 +138,8 @@ struct FusedKey {
     int rec;        // 1: records gain decisions (LaunchArgs::decisions); exists for filter
                     // type 1 with a symmetric sigma and type 3 in FP32, one item per
                     // workgroup
+    int reuse;      // 1: cross-frame form (LaunchArgs::reuse_k); separable, radius >= 1,
+                    // one item per workgroup, no decision record
 };
 
 // Sample type codes.
@@ -157,6 +168,8 @@ hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream);
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_items, hipStream_t stream);
 hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds_bytes, hipStream_t stream);
 size_t fused_lds_bytes(int radius, int region_w, int region_h);
+// LDS of the cross-frame form: reuse_k rings of ceil(16 / S) bands of S rows.
+size_t fused_reuse_lds_bytes(int radius, int region_w, int block_step, int reuse_k);
 size_t backward_lds_bytes(int radius, int region_w, int region_h);
 
 } // namespace dft2

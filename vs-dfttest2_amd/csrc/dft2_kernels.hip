@@ -45,7 +45,7 @@ __device__ __forceinline__ void store_samples4(Tout* p, const float (&v)[4], con
 template <typename Tout>
 __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
     const int item = blockIdx.x;
-    const JobDev& job = a.jobs[sgpr(find_job<false>(a.jobs, a.n_jobs, item))];
+    const JobDev& job = a.jobs[sgpr(find_job<0>(a.jobs, a.n_jobs, item))];
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles_y = sgpr(job.tiles_y);
     const int tiles = tiles_x * tiles_y;
@@ -208,9 +208,17 @@ size_t fused_lds_bytes(int radius, int region_w, int region_h) {
     return sizeof(float) * (static_cast<size_t>(region_w) * region_h + (radius + 1) * 4 * 64 * 4 + kWaves * wbuf);
 }
 
+size_t fused_reuse_lds_bytes(int radius, int region_w, int block_step, int reuse_k) {
+    const int t_len = 2 * radius + 1;
+    const int wbuf = (t_len * 256 > 448) ? t_len * 256 : 448; // as kWbuf in the fused kernel
+    const int ring_rows = (16 + block_step - 1) / block_step * block_step;
+    return sizeof(float) * (static_cast<size_t>(reuse_k) * ring_rows * region_w + (radius + 1) * 4 * 64 * 4 +
+                            kWaves * wbuf);
+}
+
 hipError_t launch_fused(const LaunchArgs& a, const FusedKey& k, unsigned grid, size_t lds_bytes,
                         hipStream_t stream) {
-    if (k.persistent != (a.queue != nullptr) || k.asym != a.asym) {
+    if (k.persistent != (a.queue != nullptr) || k.asym != a.asym || (k.reuse != 0) != (a.reuse_k > 1)) {
         return hipErrorInvalidValue;
     }
     const void* kern = fused_ptr(k);

@@ -116,7 +116,10 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
                     n_jobs);
     }
     JobDev jd[kMaxNoAllocJobs];
-    const TileShape& ts = p->tile;
+    // The plan's form, as dft2_process_batch takes it: one frame per work item, or the
+    // cross-frame form (reuse_k frames per item, its own tile).
+    const int reuse_k = p->reuse_k;
+    const TileShape& ts = reuse_k > 1 ? p->reuse_tile : p->tile;
     long long per_frame_items = 0;
     int max_frames = 0;
     for (int i = 0; i < n_jobs; i++) {
@@ -156,10 +159,14 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
     a.n_jobs = n_jobs;
     a.jobs = table;
     a.scratch = reinterpret_cast<float*>(base + kTableBytes);
-    const FusedKey key = fused_key(p, 0, a.asym);
+    FusedKey key = fused_key(p, 0, a.asym);
+    if (reuse_k > 1) {
+        a.reuse_k = reuse_k;
+        key.reuse = 1;
+    }
     JobDev chunk[kMaxNoAllocJobs];
     for (int f0 = 0; f0 < max_frames; f0 += frames_per_chunk) {
-        long long items = 0, frames = 0;
+        long long items = 0, frames = 0, groups = 0;
         for (int k = 0; k < n_jobs; k++) {
             JobDev j = jd[k];
             const int cnt = std::max(0, std::min(frames_per_chunk, j.f_count - f0));
@@ -167,8 +174,10 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
             j.f_count = cnt;
             j.item_begin = items;
             j.frame_begin = frames;
+            j.group_begin = groups;
             items += static_cast<long long>(cnt) * j.tiles_x * j.tiles_y;
             frames += cnt;
+            groups += static_cast<long long>((cnt + reuse_k - 1) / reuse_k) * j.tiles_x * j.tiles_y;
             chunk[k] = j;
         }
         if (items == 0) {
@@ -180,7 +189,7 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
         if (rc) {
             return rc;
         }
-        HIP_TRY(launch_fused(a, key, static_cast<unsigned>(items), ts.lds, st));
+        HIP_TRY(launch_fused(a, key, static_cast<unsigned>(reuse_k > 1 ? groups : items), ts.lds, st));
         HIP_TRY(launch_finalize(a, p->sample_code, items, st));
     }
     if (!stream) {
