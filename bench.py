@@ -242,7 +242,15 @@ def bench_precision4k(args):
     gpu = {}
     for prec in ("fp32", "split_bf16"):
         plan = get_plan(params, "int", 10, Backend.MI355X(precision=prec))
-        gpu[prec] = [run_with_decisions(plan, pl, params) for pl in check]
+        runs = []
+        for pl in check:
+            out_rec, dec = run_with_decisions(plan, pl, params)
+            # the product path (the cross-frame form at this tbsize) on the same clip
+            x = torch.from_numpy(pl).cuda()
+            y = torch.empty_like(x)
+            plan.process_batch([plane_job(x, y)], None)
+            runs.append((out_rec, dec, y.cpu().numpy()))
+        gpu[prec] = runs
     return res, (params, check, gpu)
 
 
@@ -386,14 +394,15 @@ def cpu_leg(planes_1080p, gpu_1080p, check4k, do_time=True):
         par = {}
         for prec, runs in gpu.items():
             worst, worst_replay, differ, total, flips, close = 0, 0, 0, 0, 0, 0.0
-            for pl, (got, dec_gpu) in zip(check, runs):
+            for pl, (got_rec, dec_gpu, got) in zip(check, runs):
                 ref = onp.process_plane(pl, params4k, 10, False)
                 d = np.abs(got.astype(np.int64) - ref.astype(np.int64))
                 worst = max(worst, int(d.max()))
                 differ += int((d > 0).sum())
                 total += d.size
-                rep = onp.process_plane(pl, params4k, 10, False, decisions=dec_gpu)
-                worst_replay = max(worst_replay, int(np.abs(got.astype(np.int64) - rep.astype(np.int64)).max()))
+                rep = onp.process_plane(pl, params4k, 10, False, decisions=dec_gpu).astype(np.int64)
+                worst_replay = max(worst_replay, int(np.abs(got.astype(np.int64) - rep).max()),
+                                   int(np.abs(got_rec.astype(np.int64) - rep).max()))
                 for (dec, near, c, _, _), dg in zip(onp.hard_decisions(pl, params4k, 10, False,
                                                                        onp.COEF_ERROR_BOUND[prec]), dec_gpu):
                     miss = dec != dg
@@ -402,6 +411,8 @@ def cpu_leg(planes_1080p, gpu_1080p, check4k, do_time=True):
                         close = max(close, float(c[miss].max()))
             par[prec] = {"max_lsb_vs_fp64": worst, "samples_differing": differ, "samples": total,
                          "max_lsb_vs_replay": worst_replay, "decisions_differing": flips,
+                         "compared": "product path (process_batch) vs FP64; it and the decision-recording run "
+                                     "vs the FP64 replay of the recorded decisions",
                          "largest_psd_distance_over_margin": round(close, 4),
                          "clip": "YUV420P10 640x360 + 2x320x180, 8 frames, same generator and filter"}
         out["parity_4k_check"] = par

@@ -3,8 +3,10 @@
     python vs-dfttest2_amd/build_native.py
 
 Compiles the gfx950 kernels (csrc/dft2_fused.hip once per radius, dft2_backward.hip,
-dft2_kernels.hip) and the C ABI (dft2_capi.cpp, dft2_batch.cpp) with hipcc, in parallel,
-and links them into vs-dfttest2_amd/dfttest2_amd/_native/.
+dft2_kernels.hip) and the C ABI (dft2_capi.cpp, dft2_batch.cpp, dft2_noalloc.cpp) with
+hipcc, in parallel, and links them into vs-dfttest2_amd/dfttest2_amd/_native/.  Where the
+VapourSynth headers are found, it also builds the VapourSynth plugin
+(vsplugin/dfttest2_mi355x_vs.cpp) beside it; otherwise it says why not.
 """
 
 from __future__ import annotations
@@ -51,6 +53,34 @@ def run(cmd):
     subprocess.run(cmd, check=True)
 
 
+VS_PLUGIN_SRC = os.path.join(HERE, "vsplugin", "dfttest2_mi355x_vs.cpp")
+VS_PLUGIN_NAME = "libdfttest2_mi355x_vs.so"
+
+
+def find_vapoursynth_include():
+    """Directory holding VapourSynth4.h: $VS_INCLUDE, then the usual include paths."""
+    cands = [os.environ.get("VS_INCLUDE", "")] + [os.path.join(r, s) for r in ("/usr/include", "/usr/local/include")
+                                                  for s in ("vapoursynth", "")]
+    for d in cands:
+        if d and os.path.exists(os.path.join(d, "VapourSynth4.h")) and os.path.exists(os.path.join(d, "VSHelper4.h")):
+            return d
+    return None
+
+
+def build_vs_plugin(lib_path: str):
+    """The VapourSynth plugin over the C ABI, where the VapourSynth headers are present."""
+    inc = find_vapoursynth_include()
+    if inc is None:
+        print("VapourSynth4.h / VSHelper4.h not found (set VS_INCLUDE): the VapourSynth plugin "
+              f"({os.path.relpath(VS_PLUGIN_SRC, HERE)}) is not built", flush=True)
+        return None
+    cxx = shutil.which("g++") or "c++"
+    out = os.path.join(OUT_DIR, VS_PLUGIN_NAME)
+    run([cxx, "-shared", "-fPIC", "-O2", "-std=c++17", f"-I{inc}", VS_PLUGIN_SRC, "-L" + os.path.dirname(lib_path),
+         "-ldfttest2_mi355x", "-Wl,-rpath,$ORIGIN", "-o", out])
+    return out
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     obj_dir = os.path.join(HERE, "_build")
@@ -70,6 +100,7 @@ def build(verbose: bool = True) -> str:
     tmp = out + ".tmp"
     run([hc, "-shared", f"--offload-arch={ARCH}", *objs, "-o", tmp])
     os.replace(tmp, out)
+    build_vs_plugin(out)
     return out
 
 
