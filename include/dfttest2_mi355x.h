@@ -69,21 +69,25 @@ extern "C" {
  * value as HIP's hipStreamLegacy; the library passes stream 0 to HIP for it. */
 #define DFT2_DEFAULT_STREAM ((void*)1)
 
-/* Plugin parameters, with the meaning and layout of the reference DFTTest arguments. */
+/* Plugin parameters, with the meaning and layout of the reference DFTTest arguments.
+ * block_size 16 with radius 0..3 (the reference's fused backends) runs on the fused
+ * kernel; other block sizes (1..64) and radii up to 7 (the reference's FFT-library
+ * backends) on the generic block path, FP32 only, as far as one block's two complex
+ * T x B x (B/2+1) buffers fit a CU's LDS.  B below is block_size. */
 typedef struct dft2_params {
-    int radius;           /* "radius": 0..3, T = 2*radius+1 */
-    int block_size;       /* "block_size": must be 16 */
-    int block_step;       /* "block_step": 1..16 (sbsize - sosize) */
+    int radius;           /* "radius": 0..7, T = 2*radius+1 */
+    int block_size;       /* "block_size": 1..64 */
+    int block_step;       /* "block_step": 1..block_size (sbsize - sosize) */
     int filter_type;      /* "filter_type": 0..6 */
     int zero_mean;        /* "zero_mean": 0 or 1 */
-    const double* window; /* "window": T*16*16 values, [t][y][x] */
+    const double* window; /* "window": T*B*B values, [t][y][x] */
     int window_len;
-    const double* sigma;  /* "sigma": 1 value (scalar) or T*16*9 values, [t][ky][kx] */
+    const double* sigma;  /* "sigma": 1 value (scalar) or T*B*(B/2+1) values, [t][ky][kx] */
     int sigma_len;
     double sigma2;        /* "sigma2" */
     double pmin;          /* "pmin" */
     double pmax;          /* "pmax" */
-    const double* window_freq; /* "window_freq": T*16*9*2 values [t][ky][kx][re,im]; needed if zero_mean */
+    const double* window_freq; /* "window_freq": T*B*(B/2+1)*2 values [t][ky][kx][re,im]; needed if zero_mean */
     int window_freq_len;
 } dft2_params;
 
@@ -185,7 +189,10 @@ DFT2_API int dft2_process_batch_sigma(dft2_plan* plan, const dft2_job* jobs, int
  * (kt, ky, kx): for q != 0, ky = (p or 0 when p = 0) if e = 0, else (16 - p or 8 when
  * p = 0), and (kt, ky, kx) = (g, ky, q) if k = 0, ((T - g) % T, (16 - ky) % 16, q) if k = 1;
  * for q = 0, (g, ky, 0) if k = 0 and (g, ky, 8) if k = 1.  The decision of type 1 is
- * psd < sigma (dropped), of type 3 pmin <= psd <= pmax (sigma applies).  No reference
+ * psd < sigma (dropped), of type 3 pmin <= psd <= pmax (sigma applies).  Plans on the
+ * generic block path (block_size != 16 or radius > 3) decide every coefficient of the
+ * T x block_size x (block_size/2+1) spectrum: per block, ceil(that / 64) masks, bit i of
+ * the sequence = coefficient i in [kt][ky][kx] order (any sigma).  No reference
  * counterpart. */
 DFT2_API int dft2_process_batch_decisions(dft2_plan* plan, const dft2_job* jobs, int n_jobs,
                                           unsigned long long* decisions_dev, long long decisions_len, void* stream);

@@ -70,8 +70,10 @@ def _params(**over):
 @pytest.mark.parametrize(
     "over,msg",
     [
-        (dict(radius=4), "radius"),
-        (dict(block_size=8), "block_size"),
+        (dict(radius=8), "radius"),
+        (dict(block_size=65), "block_size"),
+        (dict(block_size=8), "window must have 192"),  # a 16-point window for 8-point blocks
+        (dict(block_size=8, block_step=9), "block_step"),
         (dict(block_step=0), "block_step"),
         (dict(filter_type=7), "filter_type"),
         (dict(window=np.ones(100)), "window"),
@@ -82,6 +84,16 @@ def _params(**over):
 def test_plan_create_validates_arguments(over, msg):
     with pytest.raises(_lib.Dft2Error, match=msg):
         _lib.Plan(_params(**over), _lib.DFT2_SAMPLE_INT, 8)
+
+
+def test_generic_path_limits():
+    # Block sizes other than 16 (and radius > 3) take the generic block path, whose LDS per
+    # block bounds the size: 64-point blocks at tbsize 15 do not fit.
+    p = dfttest_params(sbsize=64, sosize=48, tbsize=15)
+    with pytest.raises(_lib.Dft2Error, match="LDS"):
+        _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8)
+    with pytest.raises(_lib.Dft2Error, match="split BF16"):
+        _lib.Plan(dfttest_params(sbsize=8, sosize=4), _lib.DFT2_SAMPLE_INT, 8, precision=_lib.DFT2_PRECISION_SPLIT_BF16)
 
 
 def test_plan_create_rejects_formats():

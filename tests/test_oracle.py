@@ -48,20 +48,21 @@ def test_mirror_pad_matches_numpy_reflect(h, w, step):
 
 def brute_force_block(blk, p):
     """Textbook transform of one windowed block with explicit sums (no FFT library)."""
-    T, B = p.t_len, 16
+    T, B = p.t_len, p.block_size
+    H = B // 2 + 1
     r = p.radius
     t = np.arange(T)[:, None, None]
     y = np.arange(B)[None, :, None]
     x = np.arange(B)[None, None, :]
-    X = np.zeros((T, B, 9), dtype=complex)
+    X = np.zeros((T, B, H), dtype=complex)
     for kt in range(T):
         for ky in range(B):
-            for kx in range(9):
+            for kx in range(H):
                 ph = np.exp(-2j * np.pi * (kt * t / T + ky * y / B + kx * x / B))
                 X[kt, ky, kx] = np.sum(blk * ph)
     sigma = onp.f32(p.sigma_array())
     if p.zero_mean:
-        wf = onp.f32(p.window_freq).reshape(T, B, 9, 2)
+        wf = onp.f32(p.window_freq).reshape(T, B, H, 2)
         wf = wf[..., 0] + 1j * wf[..., 1]
         gf = X[0, 0, 0].real / wf[0, 0, 0].real
         X = X - gf * wf
@@ -70,20 +71,19 @@ def brute_force_block(blk, p):
     if p.zero_mean:
         Y = Y + gf * wf
     # Inverse over t at the centre slice, inverse over y, C2R over x: the real part of
-    # each kx column with weight 1 at kx = 0 and 8 (imaginary part dropped), 2 otherwise.
+    # each kx column with weight 1 at kx = 0 and (B even) kx = B/2, whose imaginary parts
+    # are dropped, and 2 otherwise.
     out = np.zeros((B, B))
-    wk = np.array([1.0] + [2.0] * 7 + [1.0])
     yy = np.arange(B)[:, None]
     xx = np.arange(B)[None, :]
     for kt in range(T):
         for ky in range(B):
-            for kx in range(9):
-                ph = np.exp(2j * np.pi * (kt * r / T + ky * yy / B + kx * xx / B))
-                if kx in (0, 8):
-                    zy = Y[kt, ky, kx] * np.exp(2j * np.pi * (kt * r / T + ky * yy / B))
-                    out += wk[kx] * zy.real * np.cos(2 * np.pi * kx * xx / B)
+            for kx in range(H):
+                zy = Y[kt, ky, kx] * np.exp(2j * np.pi * (kt * r / T + ky * yy / B))
+                if kx == 0 or 2 * kx == B:
+                    out += zy.real * np.cos(2 * np.pi * kx * xx / B)
                 else:
-                    out += wk[kx] * (Y[kt, ky, kx] * ph).real
+                    out += 2.0 * (zy * np.exp(2j * np.pi * kx * xx / B)).real
     return out
 
 
@@ -95,9 +95,28 @@ def brute_force_block(blk, p):
 def test_filter_blocks_matches_textbook_dft(kw):
     p = dfttest_params(**kw)
     rng = np.random.default_rng(5)
-    T = p.t_len
-    blk = rng.integers(0, 256, (T, 16, 16)) * onp.f32(p.window).reshape(T, 16, 16)
+    T, B = p.t_len, p.block_size
+    blk = rng.integers(0, 256, (T, B, B)) * onp.f32(p.window).reshape(T, B, B)
     np.testing.assert_allclose(onp.filter_blocks(blk, p), brute_force_block(blk, p), rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("kw", [dict(sbsize=8, sosize=4), dict(sbsize=9, sosize=3, ftype=1, sigma=2.0),
+                                dict(sbsize=12, sosize=8, tbsize=5, ftype=3, pmin=1.0, pmax=300.0),
+                                dict(sbsize=24, sosize=18, tbsize=1, f0beta=0.5),
+                                dict(sbsize=8, sosize=4, tbsize=9, swin=2, twin=0, ftype=4, pmin=2.0)])
+def test_filter_blocks_matches_textbook_dft_other_block_sizes(kw):
+    # Odd block sizes have no Nyquist column; tbsize 9 has four temporal pairs.
+    test_filter_blocks_matches_textbook_dft(kw)
+
+
+@pytest.mark.parametrize("sb,so,tb", [(8, 4, 3), (9, 6, 1), (12, 8, 5), (24, 16, 3), (8, 4, 9)])
+def test_identity_other_block_sizes(sb, so, tb):
+    # ftype 2 with sigma 1 returns the plane for any block size and overlap (the window is
+    # normalised per residue of the block step), as at 16.
+    p = dfttest_params(ftype=2, sigma=1.0, sbsize=sb, sosize=so, tbsize=tb)
+    rng = np.random.default_rng(sb + tb)
+    x = rng.integers(0, 256, (4, 37, 41)).astype(np.uint8)
+    np.testing.assert_array_equal(onp.process_plane(x, p, 8, False), x)
 
 
 @pytest.mark.parametrize("tbsize", [1, 3, 5, 7])

@@ -172,3 +172,20 @@ def test_decision_record_layout_round_trip(t_len):
     bad[0, 0, g, slot] ^= np.uint64(1) << np.uint64(lane)  # one copy of a doubly held coefficient
     with pytest.raises(AssertionError, match="decided differently"):
         decode(bad, t_len)
+
+
+@pytest.mark.parametrize("t_len,block_size", [(3, 8), (1, 9), (9, 16), (5, 12)])
+def test_generic_decision_record_decoding(t_len, block_size):
+    # Generic-path record: bit i of a block's mask sequence is coefficient i in
+    # [kt][ky][kx] order; the last mask is padded with zero bits.
+    from dfttest2_amd.decisions import decode_generic
+
+    h = block_size // 2 + 1
+    rng = np.random.default_rng(block_size)
+    dec = rng.integers(0, 2, (2, 3, t_len, block_size, h)).astype(bool)
+    flat = dec.reshape(2, 3, -1)
+    words = (flat.shape[-1] + 63) // 64
+    masks = np.zeros((2, 3, words), np.uint64)
+    for i in range(flat.shape[-1]):
+        masks[..., i // 64] |= flat[..., i].astype(np.uint64) << np.uint64(i % 64)
+    np.testing.assert_array_equal(decode_generic(masks, t_len, block_size), dec)

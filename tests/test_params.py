@@ -111,10 +111,24 @@ def test_filter_type_translation():
 
 @pytest.mark.parametrize("bad", [dict(ftype=5), dict(sbsize=0), dict(smode=0), dict(sosize=13), dict(tbsize=0),
                                  dict(swin=12), dict(twin=-1), dict(nlocation=[1]), dict(slocation=[0.0, 1.0, 1.0]),
-                                 dict(ssystem=2), dict(tbsize=9), dict(sbsize=8, sosize=4)])
+                                 dict(ssystem=2), dict(sbsize=8, sosize=5)])
 def test_argument_errors(bad):
     with pytest.raises(ValueError):
         dfttest_params(**bad)
+
+
+@pytest.mark.parametrize("kw", [dict(sbsize=8, sosize=4), dict(sbsize=12, sosize=8, tbsize=1), dict(tbsize=9),
+                                dict(sbsize=24, sosize=18, tbsize=5), dict(sbsize=9, sosize=3)])
+def test_other_block_sizes_and_radii(kw):
+    # The reference's DFTTest() takes any sbsize >= 1 and tbsize >= 1 (python/dfttest2.py:
+    # 176-190); the layouts scale with them: window [T][B][B], sigma [T][B][B/2+1].
+    p = dfttest_params(**kw)
+    B, T = kw.get("sbsize", 16), kw.get("tbsize", 3)
+    assert p.block_size == B and p.t_len == T and p.block_step == B - kw.get("sosize", 12)
+    assert np.asarray(p.window).size == T * B * B
+    assert np.asarray(p.window_freq).size == T * B * (B // 2 + 1) * 2
+    # wscale: the window's sum of squares, S^2 sum(wt^2) / (T B^2) with a rectangular twin
+    assert p.extra["wscale"] == pytest.approx(p.block_step ** 2 / B ** 2, rel=1e-12)
 
 
 @pytest.mark.parametrize("shape", [(16,), (16, 16), (3, 16, 16), (5, 16, 16), (4, 6, 10)])

@@ -27,6 +27,7 @@ ARCH = "gfx950"
 SOURCES = tuple(("dft2_fused.hip", [f"-DDFT2_FUSED_RADIUS={r}"], f"dft2_fused_r{r}.o") for r in range(4)) + (
     ("dft2_backward.hip", [], "dft2_backward.o"),
     ("dft2_kernels.hip", [], "dft2_kernels.o"),
+    ("dft2_generic.hip", [], "dft2_generic.o"),
     ("dft2_capi.cpp", [], "dft2_capi.o"),
     ("dft2_batch.cpp", [], "dft2_batch.o"),
     ("dft2_noalloc.cpp", [], "dft2_noalloc.o"),

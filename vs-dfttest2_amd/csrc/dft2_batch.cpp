@@ -329,6 +329,9 @@ int dft2_batch_create(dft2_plan* plan, const dft2_job* jobs, int n_jobs, const d
         return fail(DFT2_ERR_ARG, "plan and out_batch must not be NULL");
     }
     *out_batch = nullptr;
+    if (plan->generic) {
+        return fail(DFT2_ERR_ARG, "batches need block_size 16 and radius <= 3 (the fused kernel)");
+    }
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
         return fail(DFT2_ERR_ARG, "invalid job list");
     }

@@ -388,8 +388,8 @@ bool device_visible(const void* ptr) {
     return at.type != hipMemoryTypeUnregistered;
 }
 
-int check_mirror(int size, int step, const char* what) {
-    int ps = pad_size(size, 16, step);
+int check_mirror(int size, int block_size, int step, const char* what) {
+    int ps = pad_size(size, block_size, step);
     int o = (ps - size) / 2;
     if (o > size - 1 || ps - o - size > size - 1) {
         return fail(DFT2_ERR_ARG, "%s %d is too small for reflection padding of %d samples", what, size, o);
@@ -593,10 +593,127 @@ int build_gain_table(const dft2_plan* p, Slot* slot, LaunchArgs& a, bool bias, h
     return DFT2_OK;
 }
 
+// Generic path's table (layout of GenericArgs, dft2_internal.h) in FP64, rounded to float.
+void build_generic_consts(dft2_plan* p, const dft2_params* prm, std::vector<float>& c) {
+    const double pi = 3.14159265358979323846;
+    const int B = prm->block_size, H = B / 2 + 1, T = 2 * prm->radius + 1, r = prm->radius;
+    GenericLayout L = generic_layout(B, T);
+    c.assign(L.total, 0.0f);
+    for (int m = 0; m < B; m++) {
+        c[L.cos_b + m] = static_cast<float>(std::cos(2 * pi * m / B));
+        c[L.sin_b + m] = static_cast<float>(std::sin(2 * pi * m / B));
+    }
+    for (int m = 0; m < T; m++) {
+        c[L.cos_t + m] = static_cast<float>(std::cos(2 * pi * m / T));
+        c[L.sin_t + m] = static_cast<float>(std::sin(2 * pi * m / T));
+    }
+    for (int i = 0; i < T * B * B; i++) {
+        // load_block's (scale * window) in float (cpu_source/getframe_impl.cpp:130-152)
+        c[L.win + i] = p->out_scale * static_cast<float>(prm->window[i]);
+    }
+    for (int i = 0; i < B * B; i++) {
+        c[L.syn + i] = static_cast<float>(prm->window[r * B * B + i]);
+    }
+    for (int i = 0; i < T * B * H; i++) {
+        c[L.sig + i] = static_cast<float>(prm->sigma_len == 1 ? prm->sigma[0] : prm->sigma[i]);
+        if (p->zero_mean) {
+            c[L.wf + 2 * i] = static_cast<float>(prm->window_freq[2 * i]);
+            c[L.wf + 2 * i + 1] = static_cast<float>(prm->window_freq[2 * i + 1]);
+        }
+    }
+}
+
+// Generic path: the block kernel writes every block's windowed output to slot scratch,
+// the gather kernel sums and rounds, one launch pair per frame chunk.
+int run_generic(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
+                unsigned long long* decisions) {
+    ChunkPlan cp;
+    int rc = plan_chunks(p, p->tile, jobs_in, p->scratch_cap, cp);
+    if (rc || cp.chunks.empty()) {
+        return rc;
+    }
+    rc = ensure_device(reinterpret_cast<void**>(&slot->scratch), &slot->scratch_bytes, cp.scratch_bytes);
+    if (rc) {
+        return rc;
+    }
+    JobDev* d_table = nullptr;
+    rc = upload_jobs(slot, cp.table.data(), sizeof(JobDev) * cp.table.size(), stream, &d_table);
+    if (rc) {
+        return rc;
+    }
+    const GenericLayout L = generic_layout(p->block_size, p->t_len);
+    GenericArgs g{};
+    g.cos_b = p->d_consts + L.cos_b;
+    g.sin_b = p->d_consts + L.sin_b;
+    g.cos_t = p->d_consts + L.cos_t;
+    g.sin_t = p->d_consts + L.sin_t;
+    g.win = p->d_consts + L.win;
+    g.syn = p->d_consts + L.syn;
+    g.sig = p->d_consts + L.sig;
+    g.wf = p->d_consts + L.wf;
+    g.n_jobs = cp.n_jobs;
+    g.blockout = slot->scratch;
+    g.block_size = p->block_size;
+    g.half = p->block_size / 2 + 1;
+    g.t_len = p->t_len;
+    g.radius = p->radius;
+    g.block_step = p->block_step;
+    g.filter_type = p->filter_type;
+    g.zero_mean = p->zero_mean;
+    g.inv_wf0 = 1.0f / p->wf0;
+    g.sigma2 = p->sigma2;
+    g.pmin = p->pmin;
+    g.pmax = p->pmax;
+    g.out_scale = p->out_scale;
+    g.peak = p->peak;
+    g.decisions = decisions;
+    g.dec_words = generic_dec_words(p);
+    for (size_t c = 0; c < cp.chunks.size(); c++) {
+        const Chunk& ch = cp.chunks[c];
+        if (ch.items == 0) {
+            continue;
+        }
+        g.jobs = d_table + c * cp.n_jobs;
+        g.n_items = static_cast<int>(ch.items);
+        int max_w = 0, max_h = 0;
+        for (int k = 0; k < cp.n_jobs; k++) {
+            const JobDev& j = cp.table[c * cp.n_jobs + k];
+            if (j.f_count > 0) {
+                max_w = std::max(max_w, j.width);
+                max_h = std::max(max_h, j.height);
+            }
+        }
+        hipEvent_t ev[3] = {};
+        rc = timing_events(p, ev);
+        if (rc) {
+            return rc;
+        }
+        if (ev[0]) {
+            HIP_TRY(hipEventRecord(ev[0], stream));
+        }
+        HIP_TRY(launch_generic_blocks(g, p->sample_code, p->gen_waves, stream));
+        if (ev[1]) {
+            HIP_TRY(hipEventRecord(ev[1], stream));
+        }
+        HIP_TRY(launch_generic_gather(g, p->sample_code, max_w, max_h, ch.frames, stream));
+        if (ev[2]) {
+            HIP_TRY(hipEventRecord(ev[2], stream));
+        }
+        rc = timing_commit(p, ev);
+        if (rc) {
+            return rc;
+        }
+    }
+    return DFT2_OK;
+}
+
 // Runs `jobs` (already validated, device memory) on `stream` using slot scratch.
 int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipStream_t stream,
              const float* sigma_dev = nullptr, float sigma_scale = 1.0f,
              unsigned long long* decisions = nullptr) {
+    if (p->generic) {
+        return run_generic(p, slot, jobs_in, stream, decisions);
+    }
     // The cross-frame form when the plan takes it (it records no decisions).
     const int reuse_k = decisions ? 1 : p->reuse_k;
     const TileShape& ts = reuse_k > 1 ? p->reuse_tile : p->tile;
@@ -683,11 +800,11 @@ int make_job(const dft2_plan* p, const dft2_job& uj, const TileShape& ts, JobDev
     if (!uj.src || !uj.dst) {
         return fail(DFT2_ERR_ARG, "job src/dst must not be NULL");
     }
-    int rc = check_mirror(uj.width, p->block_step, "width");
+    int rc = check_mirror(uj.width, p->block_size, p->block_step, "width");
     if (rc) {
         return rc;
     }
-    rc = check_mirror(uj.height, p->block_step, "height");
+    rc = check_mirror(uj.height, p->block_size, p->block_step, "height");
     if (rc) {
         return rc;
     }
@@ -718,10 +835,10 @@ int make_job(const dft2_plan* p, const dft2_job& uj, const TileShape& ts, JobDev
     j.n_frames = uj.n_frames;
     j.f_begin = uj.frame_begin;
     j.f_count = uj.frame_end - uj.frame_begin;
-    j.pad_w = pad_size(uj.width, 16, p->block_step);
-    j.pad_h = pad_size(uj.height, 16, p->block_step);
-    j.nbx = pad_num(uj.width, 16, p->block_step);
-    j.nby = pad_num(uj.height, 16, p->block_step);
+    j.pad_w = pad_size(uj.width, p->block_size, p->block_step);
+    j.pad_h = pad_size(uj.height, p->block_size, p->block_step);
+    j.nbx = pad_num(uj.width, p->block_size, p->block_step);
+    j.nby = pad_num(uj.height, p->block_size, p->block_step);
     j.ox = (j.pad_w - uj.width) / 2;
     j.oy = (j.pad_h - uj.height) / 2;
     j.tiles_x = (j.nbx + ts.bw - 1) / ts.bw;
@@ -755,16 +872,20 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
         return fail(DFT2_ERR_ARG, "params and out_plan must not be NULL");
     }
     *out_plan = nullptr;
-    // Checks of DFTTestCreate (hiprtc_source/source.cpp:719-847).
-    if (prm->radius < 0 || prm->radius > 3) {
-        return fail(DFT2_ERR_ARG, "radius must be 0, 1, 2, or 3");
+    // Checks of DFTTestCreate (hiprtc_source/source.cpp:719-847).  The fused kernel takes
+    // block_size 16 and radius 0..3, the reference's fused-backend limits; other block sizes
+    // and radii up to kMaxGenericRadius take the generic block path, as the reference's
+    // FFT-library backends do (python/dfttest2.py:283-287 restricts only the fused ones).
+    if (prm->radius < 0 || prm->radius > kMaxGenericRadius) {
+        return fail(DFT2_ERR_ARG, "radius must be in [0, %d] (0..3 with the fused kernel)", kMaxGenericRadius);
     }
-    if (prm->block_size != 16) {
-        return fail(DFT2_ERR_ARG, "block_size must be 16");
+    if (prm->block_size < 1 || prm->block_size > kMaxGenericBlock) {
+        return fail(DFT2_ERR_ARG, "block_size must be in [1, %d] (16 with the fused kernel)", kMaxGenericBlock);
     }
-    if (prm->block_step < 1 || prm->block_step > 16) {
-        return fail(DFT2_ERR_ARG, "block_step must be in [1, 16]");
+    if (prm->block_step < 1 || prm->block_step > prm->block_size) {
+        return fail(DFT2_ERR_ARG, "block_step must be in [1, %d]", prm->block_size);
     }
+    const bool generic = prm->block_size != 16 || prm->radius > 3;
     if (prm->filter_type < 0 || prm->filter_type > 6) {
         return fail(DFT2_ERR_ARG, "filter_type must be in [0, 6]");
     }
@@ -781,15 +902,23 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
         return fail(DFT2_ERR_ARG, "split BF16 precision supports filter types 0, 1, 2, 5 and 6 (use FP32 for types 3 and 4)");
     }
     const int T = 2 * prm->radius + 1;
-    if (!prm->window || prm->window_len != T * 256) {
-        return fail(DFT2_ERR_ARG, "window must have %d elements (got %d)", T * 256, prm->window_len);
+    const int B = prm->block_size, Hx = B / 2 + 1;
+    if (!prm->window || prm->window_len != T * B * B) {
+        return fail(DFT2_ERR_ARG, "window must have %d elements (got %d)", T * B * B, prm->window_len);
     }
-    if (!prm->sigma || (prm->sigma_len != 1 && prm->sigma_len != T * 16 * 9)) {
-        return fail(DFT2_ERR_ARG, "sigma must have 1 or %d elements (got %d)", T * 16 * 9, prm->sigma_len);
+    if (!prm->sigma || (prm->sigma_len != 1 && prm->sigma_len != T * B * Hx)) {
+        return fail(DFT2_ERR_ARG, "sigma must have 1 or %d elements (got %d)", T * B * Hx, prm->sigma_len);
     }
-    if (prm->zero_mean && (!prm->window_freq || prm->window_freq_len != T * 16 * 9 * 2)) {
+    if (prm->zero_mean && (!prm->window_freq || prm->window_freq_len != T * B * Hx * 2)) {
         return fail(DFT2_ERR_ARG, "window_freq with %d elements is required when zero_mean is enabled",
-                    T * 16 * 9 * 2);
+                    T * B * Hx * 2);
+    }
+    if (generic && generic_lds_per_wave(B, T) > kGenericLdsBytes) {
+        return fail(DFT2_ERR_ARG, "block_size %d with tbsize %d needs %zu KiB of LDS per block (at most %zu)", B, T,
+                    generic_lds_per_wave(B, T) >> 10, kGenericLdsBytes >> 10);
+    }
+    if (generic && precision != DFT2_PRECISION_FP32) {
+        return fail(DFT2_ERR_ARG, "split BF16 precision needs block_size 16 and radius <= 3 (the fused kernel)");
     }
     if (sample_type == DFT2_SAMPLE_INT) {
         if (bits_per_sample < 8 || bits_per_sample > 16) {
@@ -805,6 +934,8 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
 
     auto* p = new dft2_plan();
     p->device = device_id;
+    p->generic = generic ? 1 : 0;
+    p->block_size = B;
     p->radius = prm->radius;
     p->t_len = T;
     p->block_step = prm->block_step;
@@ -843,15 +974,27 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
     }
 
     std::vector<float> consts;
-    build_consts(p, prm, consts, p->out_scale);
+    if (generic) {
+        build_generic_consts(p, prm, consts);
+    } else {
+        build_consts(p, prm, consts, p->out_scale);
+    }
 
     int rc = select_device(p);
     if (rc) {
         delete p;
         return rc;
     }
-    p->tile = choose_tile(p);
-    choose_reuse(p);
+    if (generic) {
+        p->gen_waves = static_cast<int>(std::min<size_t>(4, kGenericLdsBytes / generic_lds_per_wave(B, T)));
+        p->tile = TileShape();
+        p->tile.bw = p->tile.bh = 1;
+        p->tile.region_w = p->tile.region_h = B;
+        p->tile.lds = 0;
+    } else {
+        p->tile = choose_tile(p);
+        choose_reuse(p);
+    }
     hipError_t e = hipMalloc(reinterpret_cast<void**>(&p->d_consts), sizeof(float) * consts.size());
     if (e == hipSuccess) {
         e = hipMemcpy(p->d_consts, consts.data(), sizeof(float) * consts.size(), hipMemcpyHostToDevice);
@@ -982,6 +1125,9 @@ int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, con
     if (!p) {
         return fail(DFT2_ERR_ARG, "plan must not be NULL");
     }
+    if (sigma_dev && p->generic) {
+        return fail(DFT2_ERR_ARG, "sigma as a device array needs block_size 16 and radius <= 3 (the fused kernel)");
+    }
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
         return fail(DFT2_ERR_ARG, "invalid job list");
     }
@@ -1022,7 +1168,7 @@ int dft2_process_batch_decisions(dft2_plan* p, const dft2_job* jobs, int n_jobs,
     if (p->filter_type != 1 && p->filter_type != 3) {
         return fail(DFT2_ERR_ARG, "decision records exist for filter types 1 and 3 only");
     }
-    if (p->asym) {
+    if (p->asym && !p->generic) {
         return fail(DFT2_ERR_ARG, "decision records need a symmetric sigma");
     }
     if (p->filter_type == 3 && p->precision != DFT2_PRECISION_FP32) {
@@ -1041,7 +1187,7 @@ int dft2_process_batch_decisions(dft2_plan* p, const dft2_job* jobs, int n_jobs,
         jd[i].dec_base = blocks;
         blocks += static_cast<long long>(jd[i].f_count) * jd[i].nby * jd[i].nbx;
     }
-    const long long need = blocks * (p->radius + 1) * 4;
+    const long long need = blocks * (p->generic ? generic_dec_words(p) : (p->radius + 1) * 4);
     if (decisions_len < need) {
         return fail(DFT2_ERR_ARG, "decisions_dev holds %lld masks, %lld needed", decisions_len, need);
     }
@@ -1072,6 +1218,9 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     }
     if (p->sample_code != kF32 || p->filter_type != 0 || p->precision != DFT2_PRECISION_FP32) {
         return fail(DFT2_ERR_ARG, "the backward pass needs a float32 plan with filter_type 0 and FP32 precision");
+    }
+    if (p->generic) {
+        return fail(DFT2_ERR_ARG, "the backward pass needs block_size 16 and radius <= 3 (the fused kernel)");
     }
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
         return fail(DFT2_ERR_ARG, "invalid job list");
@@ -1200,11 +1349,11 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     if (src_stride < static_cast<long long>(width) * bps || dst_stride < static_cast<long long>(width) * bps) {
         return fail(DFT2_ERR_ARG, "row stride smaller than the row");
     }
-    int rc = check_mirror(width, p->block_step, "width");
+    int rc = check_mirror(width, p->block_size, p->block_step, "width");
     if (rc) {
         return rc;
     }
-    rc = check_mirror(height, p->block_step, "height");
+    rc = check_mirror(height, p->block_size, p->block_step, "height");
     if (rc) {
         return rc;
     }

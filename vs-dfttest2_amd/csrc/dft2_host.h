@@ -34,6 +34,32 @@ int pad_size(int size, int bs, int step);
 int pad_num(int size, int bs, int step);
 
 constexpr int kRing = 8;
+// Generic block path limits: block sizes 1..64 and radius up to 7, with at most this much
+// LDS per block (one wave's two complex T x B x (B/2+1) buffers).
+constexpr int kMaxGenericBlock = 64;
+constexpr int kMaxGenericRadius = 7;
+constexpr size_t kGenericLdsBytes = 160 * 1024 - 1024;
+
+// Offsets (floats) of the generic path's table: twiddles, window, synthesis window, sigma,
+// window spectrum (GenericArgs in dft2_internal.h).
+struct GenericLayout {
+    int cos_b, sin_b, cos_t, sin_t, win, syn, sig, wf, total;
+};
+inline GenericLayout generic_layout(int B, int T) {
+    const int H = B / 2 + 1;
+    auto up4 = [](int v) { return (v + 3) / 4 * 4; };
+    GenericLayout L{};
+    L.cos_b = 0;
+    L.sin_b = up4(L.cos_b + B);
+    L.cos_t = up4(L.sin_b + B);
+    L.sin_t = up4(L.cos_t + T);
+    L.win = up4(L.sin_t + T);
+    L.syn = up4(L.win + T * B * B);
+    L.sig = up4(L.syn + B * B);
+    L.wf = up4(L.sig + T * B * H);
+    L.total = up4(L.wf + 2 * T * B * H);
+    return L;
+}
 constexpr int kTableCache = 4;
 constexpr size_t kTimingPending = 64; // launch pairs whose timing events may be pending
 
@@ -96,6 +122,9 @@ struct ChunkPlan {
 
 struct dft2_plan {
     int device = 0;
+    int generic = 0;      // block_size != 16 or radius > 3: the generic block path
+    int block_size = 16;
+    int gen_waves = 1;    // generic path: waves (blocks) per workgroup, limited by LDS
     int radius = 1;
     int t_len = 3;
     int block_step = 4;
@@ -134,6 +163,11 @@ struct dft2_plan {
 
 namespace dft2 {
 namespace host {
+
+// Decision-record masks per block of a generic plan: T x B x (B/2+1) coefficients.
+inline int generic_dec_words(const dft2_plan* p) {
+    return (p->t_len * p->block_size * (p->block_size / 2 + 1) + 63) / 64;
+}
 
 // Kernel selection for plan p; asym = -1 takes the plan's own sigma symmetry.
 inline FusedKey fused_key(const dft2_plan* p, int persistent = 0, int asym = -1) {

@@ -125,6 +125,43 @@ struct LaunchArgs {
     int ring_bands;
 };
 
+// Generic block path (block sizes other than 16, tbsize above 7): table layout in floats,
+// built in FP64 and rounded to float, and the launch arguments.
+//   cos/sin(2 pi m / B), m < B; cos/sin(2 pi m / T), m < T;
+//   WIN [t][y][x] analysis window times the sample scale; SYN [y][x] window slice r;
+//   SIG [t][ky][kx] sigma (kx < B/2 + 1); WF [t][ky][kx][re, im] window spectrum.
+struct GenericArgs {
+    const float* cos_b;
+    const float* sin_b;
+    const float* cos_t;
+    const float* sin_t;
+    const float* win;
+    const float* syn;
+    const float* sig;
+    const float* wf;
+    const JobDev* jobs;
+    int n_jobs;
+    float* blockout;      // per block (work item): its B x B windowed centre-slice output
+    int n_items;
+    int block_size, half, t_len, radius, block_step;  // B, B/2 + 1, T, r, S
+    int filter_type, zero_mean;
+    float inv_wf0, sigma2, pmin, pmax;
+    float out_scale;
+    int peak;
+    // Gain-decision record (filter types 1 and 3; nullptr = none): per block, dec_words
+    // 64-bit masks, bit i = coefficient i in [kt][ky][kx] order (type 1: psd < sigma;
+    // type 3: pmin <= psd <= pmax); block index job.dec_base + its index within the job.
+    unsigned long long* decisions;
+    int dec_words;
+};
+// LDS per wave of the generic block kernel: two complex T x B x (B/2+1) buffers.
+inline size_t generic_lds_per_wave(int block_size, int t_len) {
+    return sizeof(float) * 4 * static_cast<size_t>(t_len) * block_size * (block_size / 2 + 1);
+}
+hipError_t launch_generic_blocks(const GenericArgs& g, int sample_code, int waves_per_group, hipStream_t stream);
+hipError_t launch_generic_gather(const GenericArgs& g, int sample_code, int max_width, int max_height, long long frames,
+                                 hipStream_t stream);
+
 // Selects the fused kernel instantiation.
 struct FusedKey {
     int radius;

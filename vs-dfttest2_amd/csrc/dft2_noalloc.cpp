@@ -111,6 +111,9 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
         return fail(DFT2_ERR_ARG, "invalid job list");
     }
+    if (p->generic) {
+        return fail(DFT2_ERR_ARG, "the allocation-free entry needs block_size 16 and radius <= 3 (the fused kernel)");
+    }
     if (n_jobs > kMaxNoAllocJobs) {
         return fail(DFT2_ERR_ARG, "the allocation-free entry takes at most %d jobs (got %d)", kMaxNoAllocJobs,
                     n_jobs);

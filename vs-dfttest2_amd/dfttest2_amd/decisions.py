@@ -60,26 +60,39 @@ def decode(masks, t_len):
     return dec
 
 
-def _pad_num(size: int, block_step: int) -> int:
-    """Blocks along one axis (cpu_source/getframe_impl.cpp:11-20)."""
-    ps = size + ((16 - size % 16) if size % 16 else 0) + max(16 - block_step, block_step) * 2
-    return (ps - 16) // block_step + 1
+def is_generic(p) -> bool:
+    """Plans the library runs on its generic block path (block_size != 16 or radius > 3)."""
+    return p.block_size != 16 or p.radius > 3
+
+
+def decode_generic(masks, t_len, block_size):
+    """Generic-path record [..., words] uint64 -> decisions [..., T, B, B/2+1]: bit i of the
+    mask sequence is coefficient i in [kt][ky][kx] order."""
+    n_coef = t_len * block_size * (block_size // 2 + 1)
+    bits = (masks[..., :, None] >> np.arange(64, dtype=np.uint64)) & np.uint64(1)
+    flat = bits.reshape(masks.shape[:-1] + (-1,))[..., :n_coef].astype(bool)
+    return flat.reshape(masks.shape[:-1] + (t_len, block_size, block_size // 2 + 1))
 
 
 def run_with_decisions(plan, x, p):
     """Filters the clip x [N, H, W] (numpy) on the GPU through the decision-recording entry;
-    returns (output, decisions per output frame [ny, nx, T, 16, 9])."""
+    returns (output, decisions per output frame [ny, nx, T, B, B/2+1])."""
     import torch
 
     n, h, w = x.shape
-    ny, nx = _pad_num(h, p.block_step), _pad_num(w, p.block_step)
+    B = p.block_size
+    ps = lambda size: size + ((B - size % B) if size % B else 0) + max(B - p.block_step, p.block_step) * 2  # noqa: E731
+    ny, nx = (ps(h) - B) // p.block_step + 1, (ps(w) - B) // p.block_step + 1
     t_len = p.t_len
+    words = (t_len * B * (B // 2 + 1) + 63) // 64 if is_generic(p) else (t_len // 2 + 1) * 4
     src = torch.from_numpy(np.ascontiguousarray(x)).cuda()
     dst = torch.empty_like(src)
-    n_masks = n * ny * nx * (t_len // 2 + 1) * 4
+    n_masks = n * ny * nx * words
     rec = torch.zeros(n_masks, dtype=torch.int64, device="cuda")
     torch.cuda.synchronize()
     plan.process_batch_decisions([plane_job(src, dst)], rec.data_ptr(), n_masks, None)
     out = dst.cpu().numpy()
-    masks = rec.cpu().numpy().view(np.uint64).reshape(n, ny, nx, t_len // 2 + 1, 4)
-    return out, list(decode(masks, t_len))
+    masks = rec.cpu().numpy().view(np.uint64)
+    if is_generic(p):
+        return out, list(decode_generic(masks.reshape(n, ny, nx, words), t_len, B))
+    return out, list(decode(masks.reshape(n, ny, nx, t_len // 2 + 1, 4), t_len))

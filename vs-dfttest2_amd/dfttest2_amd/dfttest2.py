@@ -32,10 +32,13 @@ class Backend:
 
 
 def select_backend(backend, sbsize: int, tbsize: int):
-    """python/backends.py:40-63: the fused backend needs sbsize 16 and tbsize 1, 3, 5 or 7."""
+    """python/backends.py:40-63.  The MI355X backend serves both of the reference's
+    candidate lists: sbsize 16 with tbsize 1, 3, 5 or 7 on the fused kernel (the fused
+    backends' case), and other block sizes, or tbsize up to 15, on the generic block path
+    (the case the reference gives its FFT-library backends)."""
     if backend is not None:
         return backend() if isinstance(backend, type) else backend
-    if sbsize == 16 and tbsize in (1, 3, 5, 7):
+    if 1 <= sbsize <= 64 and tbsize % 2 == 1 and 1 <= tbsize <= 15:
         return Backend.MI355X()
     raise RuntimeError("No backend are available")
 

@@ -142,11 +142,11 @@ def dfttest2_params(
     block_size = sbsize
     block_step = sbsize - sosize
 
-    # The fused backends' limits (python/dfttest2.py:283-287).
-    if radius not in range(4):
-        raise ValueError("invalid radius (tbsize)")
-    if block_size != 16:
-        raise ValueError("invalid block_size (sbsize)")
+    # python/dfttest2.py:283-287 holds the fused backends to radius 0..3 and block_size 16;
+    # the plan (dft2_plan_create) takes those on the fused kernel and the rest, as the
+    # reference's FFT-library backends do, on the generic block path, and checks its limits.
+    if radius < 0 or block_size < 1 or block_step < 1 or block_step > block_size:
+        raise ValueError("invalid tbsize, sbsize or sosize")
 
     t_len = 2 * radius + 1
     y_len = block_size
