@@ -288,17 +288,19 @@ void fused_kernel(LaunchArgs a) {
     // of the item from ring slot `slot` to the frame's scratch region and zeroes the slot.
     // Scratch region of frame f_local + k = the work item (f_local + k, tile) of the
     // one-frame form, which finalize reads.
+    // The K bands are spread over all threads of the workgroup (one float4 each), so no
+    // wave carries the copy alone into the next barrier.
     auto flush_band = [&](int b, int slot) {
         const int rows = min(S, rows_used - b * S);
         const int n4 = rows * rw / 4;
-        for (int k = 0; k < kcount; k++) {
+        for (int e = tid; e < kcount * n4; e += kThreads) {
+            const int k = e / n4;
+            const int i = e - k * n4;
             float4* src = reinterpret_cast<float4*>(acc + k * ring_floats + slot * S * rw);
             const long long region_item = sgpr(static_cast<int>(job.item_begin)) + static_cast<long long>(f_local + k) * tiles + tile;
             float4* dst = reinterpret_cast<float4*>(a.scratch + region_item * (a.region_h * rw) + b * S * rw);
-            for (int i = tid; i < n4; i += kThreads) {
-                dst[i] = src[i];
-                src[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            }
+            dst[i] = src[i];
+            src[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
     };
 
@@ -308,7 +310,7 @@ void fused_kernel(LaunchArgs a) {
     const int n_pass = (nbx + kWaves - 1) / kWaves;
     const int my_pass = nbx > wave ? (nbx - wave + kWaves - 1) / kWaves : 0; // passes with a block of this wave
     raw4<Tin> nxt[T];
-    if (my_pass > 0) {
+    if (!REUSE && my_pass > 0) {
         fetch(0, wave, frame, nxt);
     }
 
@@ -323,6 +325,11 @@ void fused_kernel(LaunchArgs a) {
         // Slices to transform: all T for the item's first frame, else the one new slice,
         // which the prefetch put in nxt[T - 1].
         const bool all = !REUSE || k == 0;
+        if (REUSE && k == 0) {
+            // The block's T slices, fetched here rather than one block ahead: a prefetch
+            // would keep T raw slices live across the middle stage of all K frames.
+            fetch(r, pass * kWaves + wave, frame, nxt);
+        }
         raw4<Tin> cur_raw[T]; // the samples as loaded (16-bit samples' exact path)
         float cur[T][4];      // and as floats (REUSE: converted where used, one slice or T)
 #pragma unroll
@@ -336,11 +343,13 @@ void fused_kernel(LaunchArgs a) {
             }
         }
         {
-            if (REUSE && k + 1 < kcount) {
+            if constexpr (REUSE) {
                 // the same block at the next frame: only its newest slice
-                const __amdgpu_buffer_rsrc_t fr1[1] = {frame_rsrc(slice_base(n + k + 1 + R), frame_bytes)};
-                raw4<Tin>(&last)[1] = *reinterpret_cast<raw4<Tin>(*)[1]>(&nxt[T - 1]);
-                fetch(r, pass * kWaves + wave, fr1, last);
+                if (k + 1 < kcount) {
+                    const __amdgpu_buffer_rsrc_t fr1[1] = {frame_rsrc(slice_base(n + k + 1 + R), frame_bytes)};
+                    raw4<Tin>(&last)[1] = *reinterpret_cast<raw4<Tin>(*)[1]>(&nxt[T - 1]);
+                    fetch(r, pass * kWaves + wave, fr1, last);
+                }
             } else {
                 const bool row_end = pass + 1 == my_pass;
                 if (!row_end || r + 1 < nby) {
