@@ -78,28 +78,35 @@ def mfma_peak(t_len: int, precision: str, exact_input: bool):
                                               f"(exact products) for the {bf16} flop/block of the x rDFT")
 
 
-def mfma_per_block(t_len: int, precision: str, sample_bytes: int) -> dict:
-    """MFMA instructions one wave issues per block in the separable-window kernel: the x
-    transform of each slice (8-bit samples: 16x16x16 BF16 over the basis parts, three in
-    FP32, two in split BF16; 16-bit samples: 16x16x32 BF16 over their two bytes, three or
-    two), the y transform of each slice and the two inverse contractions (four FP32
-    16x16x4 or three split-BF16 16x16x16 each)."""
+def mfma_per_block(slices: float, precision: str, sample_bytes: int) -> dict:
+    """MFMA instructions one wave issues per output block in the separable-window kernel,
+    given the slice spectra it computes per block (T in the one-frame form; in the
+    cross-frame form a group of K frames computes K + T - 1, which the library's timing
+    record counts): the x transform of each slice (8-bit samples: 16x16x16 BF16 over the
+    basis parts, three in FP32, two in split BF16; 16-bit samples: 16x16x32 BF16 over their
+    two bytes, three or two), the y transform of each slice and the two inverse contractions
+    (four FP32 16x16x4 or three split-BF16 16x16x16 each)."""
     split = precision == "split_bf16"
     parts = 2 if split else 3
     x_kind = "bf16_16x16x16" if sample_bytes == 1 else "bf16_16x16x32"
-    counts = {x_kind: parts * t_len}
-    other = 2 + t_len  # y per slice + two inverse contractions
+    counts = {x_kind: round(parts * slices, 3)}
+    other = 2 + slices  # y per slice + two inverse contractions
     if split:
-        counts["bf16_16x16x16"] = counts.get("bf16_16x16x16", 0) + 3 * other
+        counts["bf16_16x16x16"] = round(counts.get("bf16_16x16x16", 0) + 3 * other, 3)
     else:
-        counts["f32_16x16x4"] = 4 * other
+        counts["f32_16x16x4"] = round(4 * other, 3)
     return counts
+
+
+def slices_per_block(kt: dict, t_len: int) -> float:
+    """Slice spectra computed per output block by the timed launches (library count)."""
+    return kt["slice_spectra"] / kt["blocks"] if kt.get("blocks") else float(t_len)
 
 
 def executed(counts: dict, blocks: float, kernel_ms: float) -> dict:
     """Executed MFMA work of a launch: flops per block, the executed rate, and the share of
     the SIMDs' cycles the MFMA pipe is busy with it (issue cycles per instruction above)."""
-    flop = sum(MFMA_FLOP[k] * n for k, n in counts.items())
+    flop = round(sum(MFMA_FLOP[k] * n for k, n in counts.items()))
     cycles = sum(MFMA_CYCLES[k] * n for k, n in counts.items())
     s = kernel_ms / 1e3
     return {"mfma_per_block": counts, "flop_per_block": flop, "tflops": round(blocks * flop / s / 1e12, 2),
@@ -192,8 +199,10 @@ def bench_1080p(args, precision, planes, keep_frames):
         "mpx_s": round(WIDTH * HEIGHT * FRAMES / (ms / 1e3) / 1e6, 2),
         "kernel": "fused_kernel", "kernel_ms": round(fused_ms, 4), "finalize_ms": round(fin_ms, 4),
         "launches_per_step": launches, "tflops_logical": round(logical, 3),
-        "executed": executed(mfma_per_block(params.t_len, precision, 1), blocks / launches, fused_ms),
+        "executed": executed(mfma_per_block(slices_per_block(kt, params.t_len), precision, 1), blocks / launches,
+                             fused_ms),
     }
+    summary["executed"]["slices_per_block"] = round(slices_per_block(kt, params.t_len), 4)
     host = [o[:keep_frames].cpu().numpy() for o in outs]
     del outs
     return summary, host
@@ -230,7 +239,9 @@ def bench_precision4k(args):
                      "mpx_s": round(w * h * frames / (ms / 1e3) / 1e6, 1), "kernel_ms": round(fused_ms, 3),
                      "launches_per_step": launches, "tflops_logical": round(logical, 2),
                      "frac_logical": round(logical / peak, 4),
-                     "executed": executed(mfma_per_block(params.t_len, prec, 2), blocks / launches, fused_ms)}
+                     "executed": executed(mfma_per_block(slices_per_block(kt, params.t_len), prec, 2),
+                                          blocks / launches, fused_ms)}
+        res[prec]["executed"]["slices_per_block"] = round(slices_per_block(kt, params.t_len), 4)
         log(f"4K {prec}: {ms:.2f} ms per {frames} frames, fused {fused_ms:.2f} ms/launch")
     res["speedup_split_bf16"] = round(res["fp32"]["ms_per_step"] / res["split_bf16"]["ms_per_step"], 3)
     del outs, jobs, planes

@@ -211,6 +211,12 @@ typedef struct dft2_kernel_timing {
     double fused_ms;             /* their summed duration */
     long long finalize_launches; /* finalize kernel launches recorded */
     double finalize_ms;          /* their summed duration */
+    long long blocks;            /* output blocks (block positions x output frames) the
+                                    recorded block launches computed */
+    long long slice_spectra;     /* spatial slice spectra they computed, counted per block
+                                    position: T per output block in the one-frame form,
+                                    frames + T - 1 per group of consecutive frames in the
+                                    cross-frame form */
 } dft2_kernel_timing;
 
 /* enable = 1: from now on the plan's process calls record HIP events around every fused

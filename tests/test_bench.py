@@ -52,6 +52,10 @@ def test_mfma_counts_per_block():
     assert bench.mfma_per_block(3, "split_bf16", 1) == {"bf16_16x16x16": 21}
     assert bench.mfma_per_block(5, "fp32", 2) == {"bf16_16x16x32": 15, "f32_16x16x4": 28}
     assert bench.mfma_per_block(5, "split_bf16", 2) == {"bf16_16x16x32": 10, "bf16_16x16x16": 21}
+    # Cross-frame form: 8 output frames of tbsize 5 compute 12 slices, 1.5 per block.
+    assert bench.slices_per_block({"blocks": 8, "slice_spectra": 12}, 5) == 1.5
+    assert bench.slices_per_block({"blocks": 0, "slice_spectra": 0}, 5) == 5.0
+    assert bench.mfma_per_block(1.5, "fp32", 2) == {"bf16_16x16x32": 4.5, "f32_16x16x4": 14}
     e = bench.executed({"f32_16x16x4": 20, "bf16_16x16x16": 9}, 1e6, 1.0)
     assert e["flop_per_block"] == 20 * 2048 + 9 * 8192
     # 1e6 blocks x 784 cycles in 1 ms on 1024 SIMDs at 2.4 GHz

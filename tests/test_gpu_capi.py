@@ -96,6 +96,25 @@ def test_frame_entry_rejects_bad_geometry(src_stride, dst_stride, width, height,
     assert not out.any()  # nothing was written
 
 
+@pytest.mark.parametrize("tbsize,frames,per_position", [(3, 10, 30), (5, 10, 18), (5, 16, 24), (5, 3, 7)])
+def test_timing_counts_blocks_and_slice_spectra(tbsize, frames, per_position):
+    # One-frame form (tbsize 3): T slices per output block.  Cross-frame form (tbsize 5,
+    # 8 frames per group): frames + 4 per group of consecutive frames at each block
+    # position, so 10 frames = 12 + 6 slices, 16 = 12 + 12, 3 = 7.
+    announce("timing: blocks and slice spectra", tbsize, frames)
+    p = dfttest_params(tbsize=tbsize)
+    x = synth.float_plane(frames, 48, 64, seed=75).cuda()
+    o = torch.empty_like(x)
+    plan = _lib.Plan(p, _lib.DFT2_SAMPLE_FLOAT, 32, 0, 1, 0)
+    plan.set_timing(True)
+    plan.process_batch([plane_job(x, o)], torch.cuda.current_stream().cuda_stream)
+    t = plan.timing()
+    positions = ((48 + 24) // 4 - 3) * ((64 + 24) // 4 - 3)  # padded by 12 on each side
+    assert t["blocks"] == positions * frames
+    assert t["slice_spectra"] == positions * per_position
+    plan.close()
+
+
 def test_timing_totals_over_many_launches():
     # More launch pairs than the pending pool holds: the oldest are folded into the totals
     # as new ones come, and the totals still count every launch.

@@ -545,7 +545,8 @@ int timing_events(dft2_plan* p, hipEvent_t ev[3]) {
 // Waits for the oldest pending triple, adds its durations to the totals and returns its
 // events to the free pool.  Called with p->mu held.
 int timing_resolve_oldest(dft2_plan* p) {
-    const std::array<hipEvent_t, 3> ev = p->timing_pending.front();
+    const dft2_plan::TimingRecord rec = p->timing_pending.front();
+    const std::array<hipEvent_t, 3>& ev = rec.ev;
     p->timing_pending.pop_front();
     for (hipEvent_t e : ev) {
         p->timing_free.push_back(e);
@@ -558,16 +559,38 @@ int timing_resolve_oldest(dft2_plan* p) {
     p->timing_totals.fused_ms += f;
     p->timing_totals.finalize_launches++;
     p->timing_totals.finalize_ms += g;
+    p->timing_totals.blocks += rec.blocks;
+    p->timing_totals.slice_spectra += rec.slices;
     return DFT2_OK;
 }
 
-// Queues a recorded triple; beyond kTimingPending pending triples the oldest is resolved.
-int timing_commit(dft2_plan* p, const hipEvent_t ev[3]) {
+// Output blocks of one chunk's jobs and the slice spectra computed for them: T per
+// block in the one-frame form; in the cross-frame form each group of up to reuse_k
+// consecutive frames computes frames + T - 1 slices per block position.
+void chunk_work(const dft2_plan* p, const JobDev* jobs, int n_jobs, int reuse_k, long long& blocks,
+                long long& slices) {
+    blocks = slices = 0;
+    for (int i = 0; i < n_jobs; i++) {
+        const JobDev& j = jobs[i];
+        const long long nb = static_cast<long long>(j.nbx) * j.nby;
+        blocks += nb * j.f_count;
+        if (reuse_k > 1) {
+            const long long full = j.f_count / reuse_k, rest = j.f_count % reuse_k;
+            slices += nb * (full * (reuse_k + 2 * p->radius) + (rest ? rest + 2 * p->radius : 0));
+        } else {
+            slices += nb * j.f_count * p->t_len;
+        }
+    }
+}
+
+// Queues a recorded triple with its launch pair's work; beyond kTimingPending pending
+// triples the oldest is resolved.
+int timing_commit(dft2_plan* p, const hipEvent_t ev[3], long long blocks, long long slices) {
     if (!ev[0]) {
         return DFT2_OK;
     }
     std::lock_guard<std::mutex> lk(p->mu);
-    p->timing_pending.push_back({ev[0], ev[1], ev[2]});
+    p->timing_pending.push_back({{ev[0], ev[1], ev[2]}, blocks, slices});
     while (p->timing_pending.size() > kTimingPending) {
         int rc = timing_resolve_oldest(p);
         if (rc) {
@@ -699,7 +722,9 @@ int run_generic(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hi
         if (ev[2]) {
             HIP_TRY(hipEventRecord(ev[2], stream));
         }
-        rc = timing_commit(p, ev);
+        long long blocks = 0, slices = 0;
+        chunk_work(p, &cp.table[c * cp.n_jobs], cp.n_jobs, 1, blocks, slices);
+        rc = timing_commit(p, ev, blocks, slices);
         if (rc) {
             return rc;
         }
@@ -787,7 +812,9 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
         if (ev[2]) {
             HIP_TRY(hipEventRecord(ev[2], stream));
         }
-        rc = timing_commit(p, ev);
+        long long blocks = 0, slices = 0;
+        chunk_work(p, &cp.table[c * cp.n_jobs], cp.n_jobs, reuse_k, blocks, slices);
+        rc = timing_commit(p, ev, blocks, slices);
         if (rc) {
             return rc;
         }
@@ -1100,7 +1127,7 @@ int dft2_plan_destroy(dft2_plan* p) {
         }
     }
     for (const auto& tr : p->timing_pending) {
-        for (hipEvent_t e : tr) {
+        for (hipEvent_t e : tr.ev) {
             (void)hipEventSynchronize(e);
             (void)hipEventDestroy(e);
         }

@@ -157,7 +157,11 @@ struct dft2_plan {
     // events.  Guarded by mu.
     bool timing = false;
     std::vector<hipEvent_t> timing_free;
-    std::deque<std::array<hipEvent_t, 3>> timing_pending;
+    struct TimingRecord {
+        std::array<hipEvent_t, 3> ev;
+        long long blocks, slices; // work of the launch pair (dft2_kernel_timing)
+    };
+    std::deque<TimingRecord> timing_pending;
     dft2_kernel_timing timing_totals{};
 };
 

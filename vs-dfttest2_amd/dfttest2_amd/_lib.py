@@ -103,6 +103,8 @@ class Dft2KernelTiming(ctypes.Structure):
         ("fused_ms", ctypes.c_double),
         ("finalize_launches", ctypes.c_longlong),
         ("finalize_ms", ctypes.c_double),
+        ("blocks", ctypes.c_longlong),
+        ("slice_spectra", ctypes.c_longlong),
     ]
 
 
