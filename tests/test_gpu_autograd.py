@@ -42,9 +42,15 @@ def test_diff_forward_matches_oracle(tbsize):
 
 
 @pytest.mark.parametrize("tbsize,shape,sosize", [(3, (2, 5, 40, 44), 12), (1, (1, 3, 36, 36), 12), (5, (1, 6, 34, 40), 12),
-                                                 (3, (1, 4, 38, 42), 14), (3, (1, 4, 44, 36), 8)])
+                                                 (3, (1, 4, 38, 42), 14), (3, (1, 4, 44, 36), 8),
+                                                 (3, (1, 19, 100, 40), 12), (3, (2, 9, 44, 36), 8),
+                                                 (3, (1, 10, 52, 60), 14), (5, (1, 13, 40, 44), 12),
+                                                 (7, (1, 11, 36, 36), 12)])
 def test_gradients_match_fp64_autograd(tbsize, shape, sosize):
-    # block steps 4 (default), 2 (eight blocks cover a pixel per axis) and 8
+    # block steps 4 (default), 2 (eight blocks cover a pixel per axis) and 8; tbsize >= 3
+    # takes the cross-frame backward (8 output frames per work item): clips of up to 8
+    # frames are one group, 19 frames are groups of 8, 8 and 3, 9 frames 8 and 1; the
+    # 100-row plane has two tile rows
     announce("diff backward", tbsize, shape, sosize)
     p = diff_params(tbsize=tbsize, sosize=sosize)
     b, n, h, w = shape
@@ -84,3 +90,48 @@ def test_backward_requires_wiener_float_plan():
                            width=40, height=40, n_frames=3, frame_begin=0, frame_end=3)
     with pytest.raises(_lib.Dft2Error, match="filter_type 0"):
         plan.backward_batch([job], s.data_ptr(), 1.0, s.data_ptr())
+
+
+def _grads(x0, sig0, p, clean):
+    xg = x0.clone().requires_grad_(True)
+    sg = sig0.clone().requires_grad_(True)
+    loss = torch.mean((dfttest_diff(xg, sg, p) - clean) ** 2)
+    loss.backward()
+    torch.cuda.synchronize()
+    return xg.grad.clone(), sg.grad.clone()
+
+
+@pytest.mark.parametrize("tbsize,tile_h", [(3, "16"), (3, "4"), (5, "8")])
+def test_cross_frame_backward_matches_one_frame_form(tbsize, tile_h, monkeypatch):
+    # The cross-frame backward (sums each input frame's spectrum gradients over the output
+    # frames that read it, then one adjoint spatial transform) and the one-frame kernel
+    # (one adjoint transform per slice and output frame) on the same 2 x 21 frames of
+    # 96x120, both against the FP64 autograd graph.  The two differ by more than their
+    # rounding where a coefficient's psd lies within rounding of sigma (the Wiener gain's
+    # kink, where d gain / d psd jumps), so each is held to the FP64 bar, and the
+    # cross-frame form to no worse than the one-frame form's error plus 2e-4.  Tile
+    # heights 16, 4 and 8 blocks.
+    announce("cross-frame vs one-frame backward", tbsize, tile_h)
+    p = diff_params(tbsize=tbsize)
+    b, n, h, w = 2, 21, 96, 120
+    clean = torch.stack([synth.clean_plane(n, h, w, 70 + i) for i in range(b)])
+    noisy = clean + torch.stack([synth.noise((n, h, w), 0.05, 80 + i) for i in range(b)])
+    rng = np.random.default_rng(11)
+    sig0 = torch.from_numpy(np.abs(rng.normal(8.0, 3.0, (p.t_len, 16, 9))).astype(np.float32))
+    xr = noisy.double().requires_grad_(True)
+    sr = sig0.double().requires_grad_(True)
+    wscale = p.extra["wscale"]
+    yr = torch.stack([otorch.filter_plane(xr[i], sr * wscale, p) for i in range(b)])
+    torch.mean((yr - clean.double()) ** 2).backward()
+    dev = torch.device("cuda", 0)
+    monkeypatch.setenv("DFT2_BWD_REUSE", "1")
+    gx1, gs1 = _grads(noisy.to(dev), sig0.to(dev), p, clean.to(dev))
+    monkeypatch.setenv("DFT2_BWD_REUSE", "8")
+    monkeypatch.setenv("DFT2_BWD_TILE_H", tile_h)
+    gx8, gs8 = _grads(noisy.to(dev), sig0.to(dev), p, clean.to(dev))
+    e1 = (rel_err(gx1, xr.grad), rel_err(gs1, sr.grad))
+    e8 = (rel_err(gx8, xr.grad), rel_err(gs8, sr.grad))
+    print(f"vs FP64: one-frame input {e1[0]:.2e} sigma {e1[1]:.2e}; cross-frame input {e8[0]:.2e} "
+          f"sigma {e8[1]:.2e}; between the forms {rel_err(gx8, gx1):.2e}", flush=True)
+    assert max(e1) <= 1e-3 and max(e8) <= 1e-3
+    assert e8[0] <= e1[0] + 2e-4 and e8[1] <= e1[1] + 2e-4

@@ -1252,24 +1252,72 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     if (n_jobs < 0 || (n_jobs > 0 && !jobs)) {
         return fail(DFT2_ERR_ARG, "invalid job list");
     }
-    // The backward keeps T input-gradient regions per tile in LDS, so it uses smaller
-    // tiles: 8 blocks wide and as tall as still lets two workgroups share a CU (measured
-    // on 256x256 float clips, T = 3: taller tiles amortise the per-tile work; tiles that
-    // leave one workgroup per CU are slower).
+    // Tiles are 8 blocks wide (one per wave and pass).  The one-frame form keeps T
+    // input-gradient regions per tile in LDS, so its tiles are as tall as still lets two
+    // workgroups share a CU (measured on 256x256 float clips, T = 3: taller tiles amortise
+    // the per-tile work; tiles that leave one workgroup per CU are slower).  The
+    // cross-frame form (separable window, radius >= 1) takes 8 output frames per work
+    // item and keeps ring accumulators, whose LDS does not grow with the tile height; its
+    // tiles are the tallest of 16, 8 and 4 blocks that still give every resident
+    // workgroup slot at least 4 work items (8 clips of 256x256x16, tbsize 3: 1.30 ms in
+    // the one-frame form; 8 frames per item with 16-, 8- and 4-block tiles 1.30, 1.11 and
+    // 1.09 ms, 720, 1,296 and 2,448 items for 512 slots; 4 frames per item 1.20, 1.15 and
+    // 1.24 ms).  DFT2_BWD_REUSE=K and DFT2_BWD_TILE_H=H override (K = 1: one-frame form).
     const int S = p->block_step;
     const int tbw = std::min(8, 84 / S + 1);
     auto region_w_of = [S](int tb) { return ((S * (tb - 1) + 16) + 3) / 4 * 4; };
     auto region_h_of = [S](int tb) { return S * (tb - 1) + 16; };
+    int reuse_k = p->separable && p->radius >= 1 ? 8 : 1;
+    if (const char* env = std::getenv("DFT2_BWD_REUSE")) {
+        reuse_k = p->separable && p->radius >= 1 ? std::max(1, std::min(16, std::atoi(env))) : 1;
+    }
     int tbh = tbw;
-    for (int cand : {16, 12}) {
-        const int h = std::min(cand, 84 / S + 1);
-        if (h > tbh && backward_lds_bytes(p->radius, region_w_of(tbw), region_h_of(h)) <= 80 * 1024) {
+    if (reuse_k > 1) {
+        int forced_h = 0;
+        if (const char* env = std::getenv("DFT2_BWD_TILE_H")) {
+            forced_h = std::max(1, std::min(64, std::atoi(env)));
+        }
+        int rc = select_device(p);
+        if (rc) {
+            return rc;
+        }
+        int n_cu = 0;
+        HIP_TRY(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, p->device));
+        const int wg = backward_workgroups_per_cu(p->radius, true,
+                                                  backward_reuse_lds_bytes(p->radius, region_w_of(tbw), S, reuse_k));
+        if (wg <= 0) {
+            return fail(DFT2_ERR_HIP, "the cross-frame backward kernel does not fit a CU");
+        }
+        const long long slots = static_cast<long long>(n_cu) * wg;
+        for (int h : {16, 8, 4}) {
             tbh = h;
-            break;
+            if (forced_h > 0) {
+                tbh = forced_h;
+                break;
+            }
+            long long n_items = 0;
+            for (int i = 0; i < n_jobs; i++) {
+                const int nbx = pad_num(jobs[i].width, 16, S), nby = pad_num(jobs[i].height, 16, S);
+                const long long frames = std::max(0, jobs[i].frame_end - jobs[i].frame_begin);
+                n_items += (frames + reuse_k - 1) / reuse_k * ((nbx + tbw - 1) / tbw) * ((nby + h - 1) / h);
+            }
+            if (n_items >= 4 * slots) {
+                break;
+            }
+        }
+    } else {
+        for (int cand : {16, 12}) {
+            const int h = std::min(cand, 84 / S + 1);
+            if (h > tbh && backward_lds_bytes(p->radius, region_w_of(tbw), region_h_of(h)) <= 80 * 1024) {
+                tbh = h;
+                break;
+            }
         }
     }
     const int region_w = region_w_of(tbw);
     const int region_h = region_h_of(tbh);
+    const size_t lds = reuse_k > 1 ? backward_reuse_lds_bytes(p->radius, region_w, S, reuse_k)
+                                   : backward_lds_bytes(p->radius, region_w, region_h);
     std::vector<JobDev> jd(n_jobs);
     long long items = 0;
     for (int i = 0; i < n_jobs; i++) {
@@ -1306,7 +1354,12 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
         jd[i].tiles_x = (jd[i].nbx + tbw - 1) / tbw;
         jd[i].tiles_y = (jd[i].nby + tbh - 1) / tbh;
         jd[i].item_begin = items;
-        items += static_cast<long long>(jd[i].f_count) * jd[i].tiles_x * jd[i].tiles_y;
+        jd[i].group_begin = items;
+        const long long per_tile = reuse_k > 1 ? (jd[i].f_count + reuse_k - 1) / reuse_k : jd[i].f_count;
+        items += per_tile * jd[i].tiles_x * jd[i].tiles_y;
+    }
+    if (items >= (1LL << 31)) {
+        return fail(DFT2_ERR_ARG, "too many backward work items in one call (split the jobs)");
     }
     if (items == 0) {
         return DFT2_OK;
@@ -1337,11 +1390,12 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     a.tile_bh = tbh;
     a.region_w = region_w;
     a.region_h = region_h;
+    a.reuse_k = reuse_k;
     rc = build_gain_table(p, slot, a, true, st); // sigma + 1e-15, as gain_adjoint takes it
     if (rc) {
         return rc;
     }
-    HIP_TRY(launch_backward(a, p->radius, items, backward_lds_bytes(p->radius, region_w, region_h), st));
+    HIP_TRY(launch_backward(a, p->radius, items, lds, st));
     HIP_TRY(hipEventRecord(slot->done, st));
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));

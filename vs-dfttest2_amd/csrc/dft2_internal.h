@@ -203,10 +203,17 @@ hipError_t launch_gain_table(const LaunchArgs& a, int radius, bool bias, float4*
 hipError_t launch_queue_reset(unsigned* queue, int n, hipStream_t stream);
 // Finalize over the n_items work items (frame, tile) of a fused launch.
 hipError_t launch_finalize(const LaunchArgs& a, int sample_code, long long n_items, hipStream_t stream);
+// Backward pass over n_items work items: one per (frame, tile), or (a.reuse_k > 1, separable
+// window, radius >= 1) the cross-frame form, one per (group of reuse_k frames, tile).
 hipError_t launch_backward(const LaunchArgs& a, int radius, long long n_items, size_t lds_bytes, hipStream_t stream);
 size_t fused_lds_bytes(int radius, int region_w, int region_h);
 // LDS of the cross-frame form: reuse_k rings of ceil(16 / S) bands of S rows.
 size_t fused_reuse_lds_bytes(int radius, int region_w, int block_step, int reuse_k);
 size_t backward_lds_bytes(int radius, int region_w, int region_h);
+// LDS of the cross-frame backward: reuse_k + 2 radius rings of ceil(16 / S) bands of S rows.
+size_t backward_reuse_lds_bytes(int radius, int region_w, int block_step, int reuse_k);
+// Workgroups of the backward kernel (reuse: the cross-frame form) resident per CU with
+// `lds` bytes of LDS (0 on error or when the form does not exist).
+int backward_workgroups_per_cu(int radius, bool reuse, size_t lds);
 
 } // namespace dft2
