@@ -263,11 +263,13 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
     const int l16 = lane & 15;
     const int lg = lane >> 4;
 
-    const int item = blockIdx.x;
-    const JobDev& job = a.jobs[sgpr(find_job<0>(a.jobs, a.n_jobs, item))];
+    const JobDev& job = a.jobs[sgpr(find_job<0>(a.jobs, a.n_jobs, blockIdx.x))];
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles = tiles_x * sgpr(job.tiles_y);
-    const int local = item - sgpr(static_cast<int>(job.item_begin));
+    const int job_begin = sgpr(static_cast<int>(job.item_begin));
+    // workgroup -> work item of the same job, each XCD a contiguous part of the job
+    const int item = job_xcd_item(blockIdx.x, job_begin, sgpr(job.f_count) * tiles);
+    const int local = item - job_begin;
     const int f_local = local / tiles;
     const int tile = local - f_local * tiles;
     const int ty = tile / tiles_x;
@@ -708,12 +710,14 @@ void backward_reuse_kernel(LaunchArgs a) {
     const int l16 = lane & 15;
     const int lg = lane >> 4;
 
-    const int item = blockIdx.x;
-    const JobDev& job = a.jobs[sgpr(find_job<2>(a.jobs, a.n_jobs, item))];
+    const JobDev& job = a.jobs[sgpr(find_job<2>(a.jobs, a.n_jobs, blockIdx.x))];
     const int K = sgpr(a.reuse_k);
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles = tiles_x * sgpr(job.tiles_y);
-    const int local = item - sgpr(static_cast<int>(job.group_begin));
+    const int job_begin = sgpr(static_cast<int>(job.group_begin));
+    // workgroup -> work item of the same job, each XCD a contiguous part of the job
+    const int item = job_xcd_item(blockIdx.x, job_begin, (sgpr(job.f_count) + K - 1) / K * tiles);
+    const int local = item - job_begin;
     const int group = local / tiles;
     const int tile = local - group * tiles;
     const int f_local = group * K;

@@ -254,6 +254,27 @@ __device__ __forceinline__ float gather_tail(const float* p, int k0, int lo, flo
 }
 
 
+// The work item of workgroup `bid` of a grid of n: each XCD takes a contiguous range of
+// the items, in order.  Workgroups are dispatched to the 8 XCDs round-robin by index and
+// each XCD has its own L2, so with the identity map a tile's horizontal neighbours, and
+// the same tile of the next frame, which read the same source rows, run on other XCDs
+// and fetch those rows again.  A bijection on [0, n): XCD x = bid % 8 takes items
+// x*q + min(x, n % 8) + bid / 8, q = n / 8.
+__device__ __forceinline__ int xcd_item(int bid, int n) {
+    constexpr int kXcds = 8;
+    const int q = n / kXcds;
+    const int rem = n - q * kXcds;
+    const int x = bid % kXcds;
+    return x * q + min(x, rem) + bid / kXcds;
+}
+
+// The same within one job's items [begin, begin + count), which the workgroups
+// begin .. begin + count - 1 take: each XCD takes a contiguous part of every job (plane),
+// so that planes whose tiles cost differently (luma and chroma) spread evenly.
+__device__ __forceinline__ int job_xcd_item(int bid, int begin, int count) {
+    return begin + xcd_item(bid - begin, count);
+}
+
 __device__ __forceinline__ int mirror(int v, int n) {
     v = v < 0 ? -v : v;
     return v >= n ? 2 * (n - 1) - v : v;

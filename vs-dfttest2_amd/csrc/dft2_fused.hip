@@ -188,7 +188,13 @@ void fused_kernel(LaunchArgs a) {
     const JobDev& job = a.jobs[sgpr(find_job<REUSE ? 2 : 0>(a.jobs, a.n_jobs, item))];
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles = tiles_x * sgpr(job.tiles_y);
-    const int local = item - sgpr(static_cast<int>(REUSE ? job.group_begin : job.item_begin));
+    const int job_begin = sgpr(static_cast<int>(REUSE ? job.group_begin : job.item_begin));
+    if constexpr (!PERSIST) {
+        // workgroup -> work item of the same job, each XCD a contiguous part of the job
+        const int f_count = sgpr(job.f_count);
+        item = job_xcd_item(item, job_begin, (REUSE ? (f_count + K - 1) / K : f_count) * tiles);
+    }
+    const int local = item - job_begin;
     const int group = local / tiles;
     const int tile = local - group * tiles;
     const int f_local = group * K;
