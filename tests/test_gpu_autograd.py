@@ -135,3 +135,48 @@ def test_cross_frame_backward_matches_one_frame_form(tbsize, tile_h, monkeypatch
           f"sigma {e8[1]:.2e}; between the forms {rel_err(gx8, gx1):.2e}", flush=True)
     assert max(e1) <= 1e-3 and max(e8) <= 1e-3
     assert e8[0] <= e1[0] + 2e-4 and e8[1] <= e1[1] + 2e-4
+
+
+@pytest.mark.parametrize("tbsize,reuse", [(3, "8"), (3, "1"), (5, "8")])
+def test_backward_frame_range_matches_fp64(tbsize, reuse, monkeypatch):
+    # dft2_backward_batch on output frames [4, 15) of a 20-frame clip (the C ABI's frame
+    # range): the gradient is that of a loss on those output frames only.  The output
+    # gradient comes from the FP64 forward, so the comparison isolates the backward; both
+    # backward kernels (cross-frame: 8-frame groups from frame 4, 4 + 8 = 12 and a group of
+    # 3; one-frame) against the FP64 autograd graph of the same restricted loss.
+    announce("backward frame range", tbsize, reuse)
+    from dfttest2_amd.autograd import _grad_job
+    from dfttest2_amd.dfttest2 import Backend, get_plan
+
+    p = diff_params(tbsize=tbsize)
+    n, h, w, f0, f1 = 20, 44, 52, 4, 15
+    clean = synth.clean_plane(n, h, w, 91)
+    noisy = clean + synth.noise((n, h, w), 0.05, 92)
+    rng = np.random.default_rng(13)
+    sig0 = torch.from_numpy(np.abs(rng.normal(8.0, 3.0, (p.t_len, 16, 9))).astype(np.float32))
+    wscale = p.extra["wscale"]
+    xr = noisy.double().requires_grad_(True)
+    sr = sig0.double().requires_grad_(True)
+    yr = otorch.filter_plane(xr, sr * wscale, p)
+    loss = torch.sum((yr[f0:f1] - clean.double()[f0:f1]) ** 2)
+    gy = torch.autograd.grad(loss, yr, retain_graph=True)[0]
+    loss.backward()
+
+    dev = torch.device("cuda", 0)
+    x = noisy.to(dev)
+    gyd = gy.float().to(dev)
+    gyd[:f0] = float("nan")  # outside the range: never read
+    gyd[f1:] = float("nan")
+    gx = torch.zeros_like(x)
+    sig = sig0.to(dev)
+    gs = torch.zeros_like(sig)
+    monkeypatch.setenv("DFT2_BWD_REUSE", reuse)
+    plan = get_plan(p, "float", 32, Backend.MI355X())
+    job = _grad_job(x, gyd, gx)
+    job.frame_begin, job.frame_end = f0, f1
+    plan.backward_batch([job], sig.data_ptr(), float(wscale), gs.data_ptr())
+    torch.cuda.synchronize()
+    ex, es = rel_err(gx, xr.grad), rel_err(gs, sr.grad)
+    print(f"vs FP64: input {ex:.2e}, sigma {es:.2e}", flush=True)
+    assert torch.isfinite(gx).all() and torch.isfinite(gs).all()
+    assert ex <= 1e-3 and es <= 1e-3

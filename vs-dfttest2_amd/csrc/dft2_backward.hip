@@ -43,6 +43,19 @@ __device__ __forceinline__ float wave_sum_lane0(float v) {
     return (v + r1) + (r2 + r3);
 }
 
+// A constant close to a block's samples (the wave's mean of the four samples each lane
+// holds of one slice), subtracted from every sample of the block before the forward
+// transform when zero-mean is on.  Zero-mean removes any such constant exactly: with
+// samples s - c the DC is DC - c W0 and gf is gf - c, so x = A - gf W is unchanged.  Left
+// in, the block's mean dominates the low-frequency coefficients (about the mean times the
+// window spectrum), x is a small remainder of them, and the split-BF16 contractions'
+// error relative to the coefficients becomes a large one relative to x.
+__device__ __forceinline__ float block_offset(const raw4<float>& v) {
+    const float sum = (raw4_value<float>(v, 0) + raw4_value<float>(v, 1)) + (raw4_value<float>(v, 2) + raw4_value<float>(v, 3));
+    const float total = wave_sum_lane0(sum);
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, total))) * (1.0f / 256.0f);
+}
+
 // Adjoint of one gain application: x = A - gf*W, y = g x + gf*W, where the gain is the
 // average of the Wiener gains of the coefficient's own sigma and its partner's,
 // g = (g1 + g2)/2, g_i = max(1 - c_i, 0), c_i = se_i/(psd + eps), se_i = sigma_i + eps
@@ -451,13 +464,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
     if (pass < my_pass) {
         const floatx4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
         // ---- recompute the forward spectrum, staged as [j][p][q][b*2+c] ----
+        const float boff = a.zero_mean ? block_offset(nxt[R]) : 0.0f;
         floatx4 d2[T];
 #pragma unroll
         for (int t = 0; t < T; t++) {
             float wv[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                wv[s] = win[t][s] * raw4_value<float>(nxt[t], s);
+                wv[s] = win[t][s] * (raw4_value<float>(nxt[t], s) - boff);
             }
             const Opnd in = make_opnd<kBP>(wv[0], wv[1], wv[2], wv[3]);
             const floatx4 d1 = mm16<kBP>(in, WXo, zero);
@@ -957,6 +971,7 @@ void backward_reuse_kernel(LaunchArgs a) {
     unsigned nxt_ok = 0u;
     bool nxt_full = false;
     floatx4 d2[T];     // slice spectra of output frame k (virtual frames k .. k+2R)
+    float boff = 0.0f; // sample offset of the block position (block_offset)
     floatx4 gp[2 * R]; // pending spectrum gradients of virtual frames k .. k+2R-1
 
     // The overlap-add of the pass's staged block gradients of N completed virtual frames
@@ -1036,6 +1051,8 @@ void backward_reuse_kernel(LaunchArgs a) {
             for (int i = 0; i < 2 * R; i++) {
                 gp[i] = zero;
             }
+            // one offset for every slice of this block position (see block_offset)
+            boff = a.zero_mean ? block_offset(nxt[R]) : 0.0f;
         }
         raw4<float> cur[T];
 #pragma unroll
@@ -1069,7 +1086,7 @@ void backward_reuse_kernel(LaunchArgs a) {
             float sv[4];
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                sv[s] = raw4_value<float>(cur[t], s);
+                sv[s] = raw4_value<float>(cur[t], s) - boff;
             }
             const floatx4 d1 = mm16<kBP>(make_opnd<kBP>(sv[0], sv[1], sv[2], sv[3]), basis(kBaseWX), zero);
             d2[t] = mm16<kBP>(basis(kBaseWY), make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
