@@ -44,12 +44,16 @@ __device__ __forceinline__ void store_samples4(Tout* p, const float (&v)[4], con
 // left, own), then converted to the sample type.  Scratch region index = work item.
 template <typename Tout>
 __global__ __launch_bounds__(256) void finalize_kernel(LaunchArgs a) {
-    const int item = blockIdx.x;
-    const JobDev& job = a.jobs[sgpr(find_job<0>(a.jobs, a.n_jobs, item))];
+    const JobDev& job = a.jobs[sgpr(find_job<0>(a.jobs, a.n_jobs, blockIdx.x))];
     const int tiles_x = sgpr(job.tiles_x);
     const int tiles_y = sgpr(job.tiles_y);
     const int tiles = tiles_x * tiles_y;
-    const int local = item - sgpr(static_cast<int>(job.item_begin));
+    const int job_begin = sgpr(static_cast<int>(job.item_begin));
+    // workgroup -> work item of the same job, each XCD a contiguous part of the job: the
+    // neighbours whose overlap strips a tile reads were finalized just before it on the
+    // same XCD, so the partly used lines they share are still in its L2
+    const int item = job_xcd_item(blockIdx.x, job_begin, sgpr(job.f_count) * tiles);
+    const int local = item - job_begin;
     const int f_local = local / tiles;
     const int tile = local - f_local * tiles;
     const int ty = tile / tiles_x;
