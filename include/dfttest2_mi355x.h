@@ -200,7 +200,12 @@ DFT2_API int dft2_process_batch_decisions(dft2_plan* plan, const dft2_job* jobs,
 /* Backward pass of dft2_process_batch_sigma for the Wiener filter (filter_type 0) on
  * float32 planes: accumulates d loss / d src into each job's grad_in and
  * d loss / d sigma_dev into sigma_grad_dev (T*16*9 floats, device, +=), given grad_out.
- * Block spectra are recomputed, not stored.  No reference counterpart. */
+ * Block spectra are recomputed, not stored.  grad_in is accumulated with atomic adds, so
+ * the order of its sums (not its value beyond rounding) varies from call to call.  With a
+ * separable window and radius >= 1 a work item covers 8 consecutive output frames (the
+ * environment variables DFT2_BWD_REUSE=K, K = 1 for one frame per item, and
+ * DFT2_BWD_TILE_H=H override the frames per item and the tile height in blocks).
+ * No reference counterpart. */
 DFT2_API int dft2_backward_batch(dft2_plan* plan, const dft2_grad_job* jobs, int n_jobs, const float* sigma_dev,
                         float sigma_scale, float* sigma_grad_dev, void* stream);
 
