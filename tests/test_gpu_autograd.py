@@ -45,12 +45,13 @@ def test_diff_forward_matches_oracle(tbsize):
                                                  (3, (1, 4, 38, 42), 14), (3, (1, 4, 44, 36), 8),
                                                  (3, (1, 19, 100, 40), 12), (3, (2, 9, 44, 36), 8),
                                                  (3, (1, 10, 52, 60), 14), (5, (1, 13, 40, 44), 12),
-                                                 (7, (1, 11, 36, 36), 12)])
+                                                 (7, (1, 11, 36, 36), 12), (3, (1, 1, 36, 40), 12),
+                                                 (3, (2, 2, 36, 40), 12), (5, (1, 2, 36, 40), 12)])
 def test_gradients_match_fp64_autograd(tbsize, shape, sosize):
     # block steps 4 (default), 2 (eight blocks cover a pixel per axis) and 8; tbsize >= 3
     # takes the cross-frame backward (8 output frames per work item): clips of up to 8
     # frames are one group, 19 frames are groups of 8, 8 and 3, 9 frames 8 and 1; the
-    # 100-row plane has two tile rows
+    # 100-row plane has two tile rows; clips of 1 and 2 frames clamp most slices
     announce("diff backward", tbsize, shape, sosize)
     p = diff_params(tbsize=tbsize, sosize=sosize)
     b, n, h, w = shape
