@@ -572,7 +572,8 @@ def bench_frame_host(args):
     plane on host planes (the T source frames copied in, the output copied back), as
     DFTTestGetFrame runs it (hiprtc_source/source.cpp:445-692).  1080p YUV420 u8, 48 frames;
     one caller thread on one stream slot, and four threads on four slots (VapourSynth's
-    fmParallel); pageable (numpy) and pinned (hipHostMalloc through torch) host memory.
+    fmParallel); pageable (numpy) and pinned (hipHostMalloc through torch) host memory; and
+    the same through dft2_process_frame_keyed, which uploads each source plane once.
     PCIe-inclusive frames per second; never the headline value."""
     import threading
 
@@ -597,31 +598,40 @@ def bench_frame_host(args):
             outs = [np.empty_like(a) for a in planes]
             ptr = lambda a, n: a[n].ctypes.data  # noqa: E731
 
-        def run(plan, frame_ids):
+        def run(plan, frame_ids, keyed):
             for n in frame_ids:
-                for pl, out in zip(planes, outs):
+                for k, (pl, out) in enumerate(zip(planes, outs)):
                     _, h, w = pl.shape
-                    srcs = [ptr(pl, min(max(n - r + t, 0), frames - 1)) for t in range(t_len)]
-                    plan.process_frame(srcs, w, ptr(out, n), w, w, h, memory=1)
+                    idx = [min(max(n - r + t, 0), frames - 1) for t in range(t_len)]
+                    srcs = [ptr(pl, i) for i in idx]
+                    if keyed:  # dft2_process_frame_keyed: key = (frame, plane)
+                        plan.process_frame_keyed(srcs, [4 * i + k for i in idx], w, ptr(out, n), w, w, h)
+                    else:
+                        plan.process_frame(srcs, w, ptr(out, n), w, w, h, memory=1)
 
-        for threads in (1, 4):
-            plan = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8, 0, threads, 0)
-            run(plan, range(4))
-            t0 = time.perf_counter()
-            if threads == 1:
-                run(plan, range(frames))
-            else:
-                pool = [threading.Thread(target=run, args=(plan, range(i, frames, threads))) for i in range(threads)]
-                for th in pool:
-                    th.start()
-                for th in pool:
-                    th.join()
-            dt = time.perf_counter() - t0
-            plan.close()
-            results[f"{memory}_{threads}thread"] = {"fps": round(frames / dt, 1), "ms_per_frame": round(dt / frames * 1e3, 3)}
-            log(f"frame_host {memory}, {threads} thread(s): {frames / dt:.1f} fps")
-    print(json.dumps({"metric": METRIC, "workload": "frame_host: dft2_process_frame per frame and plane on host "
-                      "planes, 1080p YUV420 u8, 48 frames, DFTTest defaults", "unit": "fps",
+        for keyed in (False, True):
+            for threads in (1, 4):
+                plan = _lib.Plan(p, _lib.DFT2_SAMPLE_INT, 8, 0, threads, 0)
+                run(plan, range(4), keyed)
+                if keyed:
+                    plan.forget_frames()
+                t0 = time.perf_counter()
+                if threads == 1:
+                    run(plan, range(frames), keyed)
+                else:
+                    pool = [threading.Thread(target=run, args=(plan, range(i, frames, threads), keyed))
+                            for i in range(threads)]
+                    for th in pool:
+                        th.start()
+                    for th in pool:
+                        th.join()
+                dt = time.perf_counter() - t0
+                plan.close()
+                name = f"{memory}_{threads}thread" + ("_keyed" if keyed else "")
+                results[name] = {"fps": round(frames / dt, 1), "ms_per_frame": round(dt / frames * 1e3, 3)}
+                log(f"frame_host {memory}, {threads} thread(s){', keyed' if keyed else ''}: {frames / dt:.1f} fps")
+    print(json.dumps({"metric": METRIC, "workload": "frame_host: dft2_process_frame (and _keyed) per frame and plane "
+                      "on host planes, 1080p YUV420 u8, 48 frames, DFTTest defaults", "unit": "fps",
                       "results": results}), flush=True)
     return 0
 

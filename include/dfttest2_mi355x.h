@@ -13,6 +13,10 @@
  *   dft2_plan_destroy   <- DFTTestFree, hiprtc_source/source.cpp:694-707
  *   dft2_process_frame  <- DFTTestGetFrame, hiprtc_source/source.cpp:445-692 (one
  *                          output frame of one plane from its T source frames)
+ *   dft2_process_frame_keyed
+ *                       <- the same on host planes, with the source planes' device
+ *                          copies kept between calls (the reference uploads T planes
+ *                          per call, hiprtc_source/source.cpp:570-573)
  *   dft2_process_batch  <- the same per-frame work over whole clips resident in device
  *                          memory, many clips/planes per launch (no reference
  *                          counterpart; the reference launches per frame and plane)
@@ -155,6 +159,26 @@ DFT2_API int dft2_plan_destroy(dft2_plan* plan);
  * directly. */
 DFT2_API int dft2_process_frame(dft2_plan* plan, const void* const* src_frames, long long src_stride, void* dst,
                        long long dst_stride, int width, int height, int memory, void* stream);
+
+/* dft2_process_frame on host planes (memory = 1) that keeps device copies of the source
+ * planes between calls.  src_keys[t] names the content of src_frames[t]: the caller
+ * guarantees that a key always names the same width x height samples for the life of the
+ * plan (or until dft2_plan_forget_frames), as (frame number, plane) does for a
+ * VapourSynth clip, whose frames are immutable.  A plane whose key an earlier call
+ * uploaded is not copied again, so consecutive output frames upload one new source plane
+ * each instead of T.  The copies are shared by the plan's stream slots and calls from
+ * several threads; the plan keeps 3 * (T + 1) * num_streams of them, enough for three
+ * planes of consecutive frames per stream slot (least recently used ones are replaced).
+ * Returns when dst is written.  Replaces the per-call uploads of the
+ * reference's GetFrame (hiprtc_source/source.cpp:570-573), which have no cache. */
+DFT2_API int dft2_process_frame_keyed(dft2_plan* plan, const void* const* src_frames,
+                                      const unsigned long long* src_keys, long long src_stride, void* dst,
+                                      long long dst_stride, int width, int height);
+
+/* Forgets every source plane dft2_process_frame_keyed holds: later calls upload them
+ * again whatever their keys (their device memory is kept for reuse).  No reference
+ * counterpart. */
+DFT2_API int dft2_plan_forget_frames(dft2_plan* plan);
 
 /* Filters every job (device memory) with one fused launch per frame chunk, queued on
  * `stream` (NULL = the plan's own stream, and the call waits for completion). */

@@ -37,7 +37,7 @@ def test_plugin_registers_the_reference_functions():
     assert reg["Version"] == ("", "any")
     text = open(SRC).read()
     assert '"dfttest2_mi355x"' in text  # the namespace the front-end diff selects
-    for call in ("dft2_plan_create", "dft2_process_frame", "dft2_plan_destroy", "dft2_rdft", "dft2_version"):
+    for call in ("dft2_plan_create", "dft2_process_frame_keyed", "dft2_plan_destroy", "dft2_rdft", "dft2_version"):
         assert call in text
 
 

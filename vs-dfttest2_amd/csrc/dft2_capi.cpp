@@ -1135,6 +1135,14 @@ int dft2_plan_destroy(dft2_plan* p) {
     for (hipEvent_t e : p->timing_free) {
         (void)hipEventDestroy(e);
     }
+    for (auto& fp : p->frame_planes) {
+        if (fp.ready) {
+            (void)hipEventDestroy(fp.ready);
+        }
+        if (fp.dev) {
+            (void)hipFree(fp.dev);
+        }
+    }
     if (p->d_consts) {
         (void)hipFree(p->d_consts);
     }
@@ -1403,24 +1411,18 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     return DFT2_OK;
 }
 
-int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long src_stride, void* dst,
-                       long long dst_stride, int width, int height, int memory, void* stream) {
-    clear_error();
-    if (!p || !src_frames || !dst) {
-        return fail(DFT2_ERR_ARG, "plan, src_frames and dst must not be NULL");
-    }
-    const int T = p->t_len;
-    for (int t = 0; t < T; t++) {
+namespace {
+// Checks of a per-frame call (dft2_process_frame, dft2_process_frame_keyed): the source
+// planes and the caller's geometry, before anything is staged or copied (the host paths
+// copy height rows of width samples at the caller's strides).
+int check_frame_call(const dft2_plan* p, const void* const* src_frames, long long src_stride, long long dst_stride,
+                     int width, int height) {
+    for (int t = 0; t < p->t_len; t++) {
         if (!src_frames[t]) {
             return fail(DFT2_ERR_ARG, "src_frames[%d] is NULL", t);
         }
     }
     const int bps = p->bytes_per_sample;
-    if (memory != 0 && memory != 1) {
-        return fail(DFT2_ERR_ARG, "memory must be 0 (device) or 1 (host)");
-    }
-    // The caller's geometry, checked before anything is staged or copied (the host path
-    // copies height rows of width samples at the caller's strides).
     if (width < 1 || height < 1) {
         return fail(DFT2_ERR_ARG, "width and height must be positive (got %d x %d)", width, height);
     }
@@ -1434,7 +1436,50 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     if (rc) {
         return rc;
     }
-    rc = check_mirror(height, p->block_size, p->block_step, "height");
+    return check_mirror(height, p->block_size, p->block_step, "height");
+}
+
+// The source planes a dft2_process_frame_keyed call uses, released when the call ends
+// (its work is complete then: the call waits for its stream).  A plane this call was to
+// upload and did not is invalidated.
+struct FramePlaneClaim {
+    dft2_plan* p;
+    FramePlane* use[kMaxFramePtrs] = {};
+    bool mine[kMaxFramePtrs] = {};
+    int n = 0;
+    ~FramePlaneClaim() {
+        {
+            std::lock_guard<std::mutex> lk(p->mu);
+            for (int t = 0; t < n; t++) {
+                bool first = true;
+                for (int u = 0; u < t; u++) {
+                    first = first && use[u] != use[t];
+                }
+                if (!use[t] || !first) {
+                    continue;
+                }
+                if (mine[t] && !use[t]->recorded) {
+                    use[t]->valid = false;
+                }
+                use[t]->users--;
+            }
+        }
+    }
+};
+} // namespace
+
+int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long src_stride, void* dst,
+                       long long dst_stride, int width, int height, int memory, void* stream) {
+    clear_error();
+    if (!p || !src_frames || !dst) {
+        return fail(DFT2_ERR_ARG, "plan, src_frames and dst must not be NULL");
+    }
+    const int T = p->t_len;
+    const int bps = p->bytes_per_sample;
+    if (memory != 0 && memory != 1) {
+        return fail(DFT2_ERR_ARG, "memory must be 0 (device) or 1 (host)");
+    }
+    int rc = check_frame_call(p, src_frames, src_stride, dst_stride, width, height);
     if (rc) {
         return rc;
     }
@@ -1550,6 +1595,199 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         for (int y = 0; y < height; y++) {
             std::memcpy(out + y * dst_stride, h_stage + T * plane + y * row, static_cast<size_t>(row));
         }
+    }
+    return DFT2_OK;
+}
+
+int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const unsigned long long* src_keys,
+                             long long src_stride, void* dst, long long dst_stride, int width, int height) {
+    clear_error();
+    if (!p || !src_frames || !src_keys || !dst) {
+        return fail(DFT2_ERR_ARG, "plan, src_frames, src_keys and dst must not be NULL");
+    }
+    int rc = check_frame_call(p, src_frames, src_stride, dst_stride, width, height);
+    if (rc) {
+        return rc;
+    }
+    const int T = p->t_len;
+    const long long row = static_cast<long long>(width) * p->bytes_per_sample;
+    const long long plane = row * height;
+    rc = select_device(p);
+    if (rc) {
+        return rc;
+    }
+    const bool pinned_dst = device_visible(dst);
+    Slot* slot = acquire_slot(p);
+    SlotGuard guard{p, slot};
+    hipStream_t st = slot->stream;
+    // The slot's previous call may have run on a caller's stream: wait for it before the
+    // staging buffers and scratch are reused.
+    HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    HIP_TRY(hipStreamSynchronize(st));
+    rc = ensure_device(&slot->stage, &slot->stage_bytes, static_cast<size_t>(plane));
+    if (rc) {
+        return rc;
+    }
+    rc = ensure_host(&slot->h_stage, &slot->h_stage_bytes, static_cast<size_t>(plane) * (T + 1));
+    if (rc) {
+        return rc;
+    }
+    char* h_stage = static_cast<char*>(slot->h_stage);
+
+    // Claim a device plane per source: one holding its key whose upload is ordered before
+    // this call's work (queued on this call's stream, or already complete), or (a miss)
+    // the least recently used plane no call uses, which this call then uploads.  A plane
+    // another call is still uploading on another stream is not waited for: this call
+    // uploads its own copy (waiting, on the host for the upload to be queued and on the
+    // GPU across streams, made concurrent calls slower than uploading again).  The same
+    // key twice (the temporal clamp at a clip's ends) shares one plane.  The pool holds
+    // 3 (T + 1) planes per stream slot, so that a caller going through consecutive frames
+    // of three planes (YUV) finds the T - 1 planes it uploaded for each plane's previous
+    // frame; a call holds at most T, so a free plane always exists.
+    FramePlaneClaim claim{p};
+    claim.n = T;
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        if (p->frame_planes.empty()) {
+            p->frame_planes.resize(static_cast<size_t>(3 * (T + 1)) * p->slots.size());
+        }
+        for (int t = 0; t < T; t++) {
+            for (int u = 0; u < t && !claim.use[t]; u++) {
+                if (src_keys[u] == src_keys[t]) {
+                    claim.use[t] = claim.use[u];
+                    claim.mine[t] = claim.mine[u];
+                }
+            }
+            if (claim.use[t]) {
+                continue;
+            }
+            FramePlane* hit = nullptr;
+            FramePlane* victim = nullptr;
+            for (auto& fp : p->frame_planes) {
+                if (fp.valid && !fp.stale && fp.key == src_keys[t] && fp.width == width && fp.height == height &&
+                    fp.bps == p->bytes_per_sample && fp.recorded &&
+                    (fp.up_stream == st || hipEventQuery(fp.ready) == hipSuccess)) {
+                    hit = &fp;
+                    break;
+                }
+                if (fp.users == 0 &&
+                    (!victim || (victim->valid && !fp.valid) || (fp.valid == victim->valid && fp.last_use < victim->last_use))) {
+                    victim = &fp;
+                }
+            }
+            FramePlane* fp = hit ? hit : victim;
+            if (!fp) {
+                return fail(DFT2_ERR_STATE, "no free source plane in the frame cache");
+            }
+            if (!hit) {
+                fp->key = src_keys[t];
+                fp->valid = true;
+                fp->stale = false;
+                fp->recorded = false;
+                fp->width = width;
+                fp->height = height;
+                fp->bps = p->bytes_per_sample;
+                claim.mine[t] = true;
+            }
+            fp->users++;
+            fp->last_use = ++p->frame_clock;
+            claim.use[t] = fp;
+        }
+    }
+
+    // Upload this call's misses: pinned planes directly, pageable ones through the slot's
+    // pinned buffer (one part per plane, as dft2_process_frame stages them).
+    int n_staged = 0;
+    for (int t = 0; t < T; t++) {
+        bool first = true;
+        for (int u = 0; u < t; u++) {
+            first = first && claim.use[u] != claim.use[t];
+        }
+        if (!claim.mine[t] || !first) {
+            continue;
+        }
+        FramePlane& fp = *claim.use[t];
+        rc = ensure_device(&fp.dev, &fp.cap, static_cast<size_t>(plane));
+        if (rc) {
+            return rc;
+        }
+        if (!fp.ready) {
+            HIP_TRY(hipEventCreateWithFlags(&fp.ready, hipEventDisableTiming));
+        }
+        if (device_visible(src_frames[t])) {
+            HIP_TRY(hipMemcpy2DAsync(fp.dev, row, src_frames[t], src_stride, row, height, hipMemcpyHostToDevice, st));
+        } else {
+            char* part = h_stage + n_staged * plane;
+            const char* src = static_cast<const char*>(src_frames[t]);
+            for (int y = 0; y < height; y++) {
+                std::memcpy(part + y * row, src + y * src_stride, static_cast<size_t>(row));
+            }
+            HIP_TRY(hipMemcpyAsync(fp.dev, part, static_cast<size_t>(plane), hipMemcpyHostToDevice, st));
+            n_staged++;
+        }
+        HIP_TRY(hipEventRecord(fp.ready, st));
+        fp.up_stream = st;
+    }
+    // Publish them: later calls may share them from now on.
+    {
+        std::lock_guard<std::mutex> lk(p->mu);
+        for (int t = 0; t < T; t++) {
+            if (claim.mine[t]) {
+                claim.use[t]->recorded = true;
+            }
+        }
+    }
+
+    // The filter on the device planes (as dft2_process_frame with device pointers), the
+    // output staged through the slot's device buffer.
+    dft2_job uj{};
+    uj.src = claim.use[0]->dev;
+    uj.dst = slot->stage;
+    uj.width = width;
+    uj.height = height;
+    uj.n_frames = T;
+    uj.frame_begin = p->radius;
+    uj.frame_end = p->radius + 1;
+    uj.src_stride = row;
+    uj.dst_stride = row;
+    JobDev j;
+    rc = make_job(p, uj, p->tile, j);
+    if (rc) {
+        return rc;
+    }
+    j.use_fptr = 1;
+    for (int t = 0; t < T; t++) {
+        j.fptr[t] = claim.use[t]->dev;
+    }
+    j.dst_fp = 0;
+    rc = run_jobs(p, slot, {j}, st);
+    if (rc) {
+        return rc;
+    }
+    if (pinned_dst) {
+        HIP_TRY(hipMemcpy2DAsync(dst, dst_stride, slot->stage, row, row, height, hipMemcpyDeviceToHost, st));
+    } else {
+        HIP_TRY(hipMemcpyAsync(h_stage + T * plane, slot->stage, static_cast<size_t>(plane), hipMemcpyDeviceToHost, st));
+    }
+    HIP_TRY(hipEventRecord(slot->done, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (!pinned_dst) {
+        char* out = static_cast<char*>(dst);
+        for (int y = 0; y < height; y++) {
+            std::memcpy(out + y * dst_stride, h_stage + T * plane + y * row, static_cast<size_t>(row));
+        }
+    }
+    return DFT2_OK;
+}
+
+int dft2_plan_forget_frames(dft2_plan* p) {
+    clear_error();
+    if (!p) {
+        return fail(DFT2_ERR_ARG, "plan must not be NULL");
+    }
+    std::lock_guard<std::mutex> lk(p->mu);
+    for (auto& fp : p->frame_planes) {
+        fp.stale = true;
     }
     return DFT2_OK;
 }

@@ -38,6 +38,7 @@ constexpr int kRing = 8;
 // LDS per block (one wave's two complex T x B x (B/2+1) buffers).
 constexpr int kMaxGenericBlock = 64;
 constexpr int kMaxGenericRadius = 7;
+static_assert(2 * kMaxGenericRadius + 1 <= kMaxFramePtrs, "a job's frame pointers must cover T source frames");
 constexpr size_t kGenericLdsBytes = 160 * 1024 - 1024;
 
 // Offsets (floats) of the generic path's table: twiddles, window, synthesis window, sigma,
@@ -92,6 +93,22 @@ struct Slot {
     void* h_stage = nullptr; // host-memory path, pageable planes: pinned copy of the same
     size_t h_stage_bytes = 0;
     bool busy = false;
+};
+
+// One device copy of a host source plane for dft2_process_frame_keyed, shared by the
+// plan's stream slots (guarded by the plan's mutex).
+struct FramePlane {
+    unsigned long long key = 0;
+    bool valid = false;      // holds (or is being given) the plane named by key, of the geometry below
+    bool stale = false;      // forgotten (dft2_plan_forget_frames): no later call matches it
+    bool recorded = false;   // its upload is queued and `ready` recorded: other calls may use it
+    int width = 0, height = 0, bps = 0;
+    int users = 0;           // calls that use the plane now; 0: it may be replaced
+    unsigned long long last_use = 0;
+    void* dev = nullptr;     // width * height * bps bytes, rows packed
+    size_t cap = 0;
+    hipEvent_t ready = nullptr; // recorded after the upload, on the uploading call's stream
+    hipStream_t up_stream = nullptr; // that stream: calls on it are ordered after the upload
 };
 
 // Tile of blocks owned by one workgroup of the fused kernel and its LDS footprint.
@@ -163,6 +180,10 @@ struct dft2_plan {
     };
     std::deque<TimingRecord> timing_pending;
     dft2_kernel_timing timing_totals{};
+    // Source planes of dft2_process_frame_keyed (2 * T * num_streams, made on first use),
+    // guarded by mu; frame_clock orders their uses for replacement.
+    std::vector<dft2::host::FramePlane> frame_planes;
+    unsigned long long frame_clock = 0;
 };
 
 namespace dft2 {

@@ -56,6 +56,9 @@ constexpr int kOffCTS = kOffAS + 8;           // up to 7, padded to 8
 constexpr int kOffWTS = kOffCTS + 8;          // up to 49, padded to 64
 constexpr int kConstFloats = kOffWTS + 64;
 
+// Source frames of one output frame at the largest radius (7, the generic block path).
+constexpr int kMaxFramePtrs = 15;
+
 // One plane of one clip in a launch (device copy of dft2_job plus derived geometry).
 struct JobDev {
     const void* src;
@@ -68,8 +71,8 @@ struct JobDev {
     int tiles_x, tiles_y;
     long long item_begin;       // first work item (fused kernel) of this job
     long long frame_begin;      // first job-frame index (finalize kernel) of this job
-    int use_fptr;               // 1: source frame f is fptr[f] (f < 7) instead of src + f*src_fp
-    const void* fptr[7];
+    int use_fptr;               // 1: source frame f is fptr[f] (f < T <= kMaxFramePtrs) instead of src + f*src_fp
+    const void* fptr[kMaxFramePtrs];
     // Backward pass only: gradient of the output (same geometry as dst) and gradient of
     // the input (same geometry as src, accumulated with +=), pitches in samples.
     const void* grad_out;

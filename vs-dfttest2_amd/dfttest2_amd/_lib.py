@@ -23,6 +23,8 @@ EXPORTED_SYMBOLS = (
     "dft2_plan_create",
     "dft2_plan_destroy",
     "dft2_process_frame",
+    "dft2_process_frame_keyed",
+    "dft2_plan_forget_frames",
     "dft2_process_batch",
     "dft2_process_batch_sigma",
     "dft2_process_batch_decisions",
@@ -172,6 +174,13 @@ def load():
         ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
     ]
     lib.dft2_process_frame.restype = ctypes.c_int
+    lib.dft2_process_frame_keyed.argtypes = [
+        ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_ulonglong), ctypes.c_longlong,
+        ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+    ]
+    lib.dft2_process_frame_keyed.restype = ctypes.c_int
+    lib.dft2_plan_forget_frames.argtypes = [ctypes.c_void_p]
+    lib.dft2_plan_forget_frames.restype = ctypes.c_int
     lib.dft2_process_batch.argtypes = [ctypes.c_void_p, ctypes.POINTER(Dft2Job), ctypes.c_int, ctypes.c_void_p]
     lib.dft2_process_batch.restype = ctypes.c_int
     lib.dft2_process_batch_sigma.argtypes = [
@@ -344,6 +353,18 @@ class Plan:
         ptrs = (ctypes.c_void_p * len(src_ptrs))(*src_ptrs)
         _check(load().dft2_process_frame(self._h, ptrs, src_stride, ctypes.c_void_p(dst), dst_stride, width, height,
                                          memory, _stream_arg(stream)))
+
+    def process_frame_keyed(self, src_ptrs: Sequence[int], keys: Sequence[int], src_stride: int, dst: int,
+                            dst_stride: int, width: int, height: int):
+        """process_frame on host planes, keeping device copies of the source planes named by
+        `keys` between calls (a key must always name the same plane content)."""
+        ptrs = (ctypes.c_void_p * len(src_ptrs))(*src_ptrs)
+        ks = (ctypes.c_ulonglong * len(keys))(*[int(k) for k in keys])
+        _check(load().dft2_process_frame_keyed(self._h, ptrs, ks, src_stride, ctypes.c_void_p(dst), dst_stride,
+                                               width, height))
+
+    def forget_frames(self):
+        _check(load().dft2_plan_forget_frames(self._h))
 
 
 class Batch:

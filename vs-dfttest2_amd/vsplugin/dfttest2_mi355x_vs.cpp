@@ -10,7 +10,9 @@
 //   DFTTestGetFrame (:445-692)                        -> GetFrame: requests the frames
 //       n - r .. n + r, clamps them at the clip ends (cpu_source/getframe_impl.cpp:294-298),
 //       copies unprocessed planes from the centre frame, and filters each processed plane
-//       with dft2_process_frame on the frames' host planes;
+//       with dft2_process_frame_keyed on the frames' host planes, keyed by (frame number,
+//       plane): frames of a node are immutable, so each source plane is uploaded once for
+//       the T output frames that read it (the reference uploads T planes per call);
 //   DFTTestFree (:694-707)                            -> Free;
 //   RDFT (:936-1007), Version                         -> dft2_rdft, dft2_version.
 // fmParallel: VapourSynth calls GetFrame from several threads; the plan admits
@@ -51,8 +53,10 @@ const VSFrame* VS_CC GetFrame(int n, int activation_reason, void* instance_data,
     }
     const int t_len = 2 * d->radius + 1;
     const VSFrame* src[7] = {};
+    int src_n[7] = {};
     for (int t = 0; t < t_len; t++) {
-        src[t] = vsapi->getFrameFilter(std::clamp(n - d->radius + t, 0, vi->numFrames - 1), d->node, frame_ctx);
+        src_n[t] = std::clamp(n - d->radius + t, 0, vi->numFrames - 1);
+        src[t] = vsapi->getFrameFilter(src_n[t], d->node, frame_ctx);
     }
     const VSFrame* centre = src[d->radius];
     const VSFrame* copy_from[3] = {d->process[0] ? nullptr : centre, d->process[1] ? nullptr : centre,
@@ -65,12 +69,14 @@ const VSFrame* VS_CC GetFrame(int n, int activation_reason, void* instance_data,
             continue;
         }
         const void* planes[7] = {};
+        unsigned long long keys[7] = {};
         for (int t = 0; t < t_len; t++) {
             planes[t] = vsapi->getReadPtr(src[t], plane);
+            keys[t] = static_cast<unsigned long long>(src_n[t]) * 4 + static_cast<unsigned long long>(plane);
         }
-        if (dft2_process_frame(d->plan, planes, vsapi->getStride(src[0], plane), vsapi->getWritePtr(dst, plane),
-                               vsapi->getStride(dst, plane), vsapi->getFrameWidth(dst, plane),
-                               vsapi->getFrameHeight(dst, plane), /*memory=*/1, nullptr) != DFT2_OK) {
+        if (dft2_process_frame_keyed(d->plan, planes, keys, vsapi->getStride(src[0], plane),
+                                     vsapi->getWritePtr(dst, plane), vsapi->getStride(dst, plane),
+                                     vsapi->getFrameWidth(dst, plane), vsapi->getFrameHeight(dst, plane)) != DFT2_OK) {
             error = dft2_last_error();
         }
     }
