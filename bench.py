@@ -286,7 +286,11 @@ def bench_diff(args):
             dfttest_diff(x, sigma, p)
 
     def fwd_bwd():
-        loss = torch.mean((dfttest_diff(x, sigma, p) - clean) ** 2)
+        # a training step: gradients released first (optimizer.zero_grad(set_to_none=True),
+        # torch's default), then forward, MSE loss and backward
+        x.grad = None
+        sigma.grad = None
+        loss = torch.nn.functional.mse_loss(dfttest_diff(x, sigma, p), clean)
         loss.backward()
 
     ms_f = _time_steps(fwd, args.steps, args.warmup, stream)

@@ -1,7 +1,9 @@
 """Differentiable path on the GPU: forward of dfttest_diff against the oracle, and the
 backward kernel's gradients (input planes and sigma) against the FP64 torch autograd
 graph of the same pipeline (oracle/dfttest_torch.py, itself pinned against finite
-differences).  Bar: relative error <= 1e-3 (norm-wise)."""
+differences).  Bar: relative error <= 1e-4 (norm-wise).  The backward kernels recompute
+the block spectra in FP32, as the forward computes them, so each gain and its derivative
+is the forward's; their linear adjoint contractions (split BF16) leave about 1e-5."""
 
 import sys
 
@@ -73,8 +75,10 @@ def test_gradients_match_fp64_autograd(tbsize, shape, sosize):
     loss_r.backward()
 
     assert float(loss.detach()) == pytest.approx(float(loss_r.detach()), rel=1e-5)
-    assert rel_err(xg.grad, xr.grad) <= 1e-3
-    assert rel_err(sg.grad, sr.grad) <= 1e-3
+    ex, es = rel_err(xg.grad, xr.grad), rel_err(sg.grad, sr.grad)
+    print(f"vs FP64: input {ex:.2e}, sigma {es:.2e}", flush=True)
+    assert ex <= 1e-4
+    assert es <= 1e-4
 
 
 def test_backward_requires_wiener_float_plan():
@@ -107,10 +111,10 @@ def test_cross_frame_backward_matches_one_frame_form(tbsize, tile_h, monkeypatch
     # The cross-frame backward (sums each input frame's spectrum gradients over the output
     # frames that read it, then one adjoint spatial transform) and the one-frame kernel
     # (one adjoint transform per slice and output frame) on the same 2 x 21 frames of
-    # 96x120, both against the FP64 autograd graph.  The two differ by more than their
-    # rounding where a coefficient's psd lies within rounding of sigma (the Wiener gain's
-    # kink, where d gain / d psd jumps), so each is held to the FP64 bar, and the
-    # cross-frame form to no worse than the one-frame form's error plus 2e-4.  Tile
+    # 96x120, both against the FP64 autograd graph and against each other.  Both recompute
+    # the spectra in FP32, as the forward does, so they take the same side of the Wiener
+    # gain's kink (psd = sigma, where d gain / d psd jumps) everywhere; with split-BF16
+    # recomputes they differed by 3e-4 where a psd lay within rounding of sigma.  Tile
     # heights 16, 4 and 8 blocks.
     announce("cross-frame vs one-frame backward", tbsize, tile_h)
     p = diff_params(tbsize=tbsize)
@@ -134,8 +138,8 @@ def test_cross_frame_backward_matches_one_frame_form(tbsize, tile_h, monkeypatch
     e8 = (rel_err(gx8, xr.grad), rel_err(gs8, sr.grad))
     print(f"vs FP64: one-frame input {e1[0]:.2e} sigma {e1[1]:.2e}; cross-frame input {e8[0]:.2e} "
           f"sigma {e8[1]:.2e}; between the forms {rel_err(gx8, gx1):.2e}", flush=True)
-    assert max(e1) <= 1e-3 and max(e8) <= 1e-3
-    assert e8[0] <= e1[0] + 2e-4 and e8[1] <= e1[1] + 2e-4
+    assert max(e1) <= 1e-4 and max(e8) <= 1e-4
+    assert rel_err(gx8, gx1) <= 1e-4 and rel_err(gs8, gs1) <= 1e-4
 
 
 @pytest.mark.parametrize("tbsize,reuse", [(3, "8"), (3, "1"), (5, "8")])
@@ -180,4 +184,4 @@ def test_backward_frame_range_matches_fp64(tbsize, reuse, monkeypatch):
     ex, es = rel_err(gx, xr.grad), rel_err(gs, sr.grad)
     print(f"vs FP64: input {ex:.2e}, sigma {es:.2e}", flush=True)
     assert torch.isfinite(gx).all() and torch.isfinite(gs).all()
-    assert ex <= 1e-3 and es <= 1e-3
+    assert ex <= 1e-4 and es <= 1e-4

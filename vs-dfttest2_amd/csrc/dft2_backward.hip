@@ -263,9 +263,10 @@ __device__ __forceinline__ void sigma_grad_out(float* dsig_lds, const float2 (&d
 template <int R>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1 ? 4 : 1))) void backward_kernel(LaunchArgs a) {
     constexpr int T = 2 * R + 1;
-    // Contractions in split BF16 (hi + lo operands, three products, FP32 accumulation):
-    // about 16 significant bits per operand, well inside the gradients' 1e-3 bound, at a
-    // fraction of the FP32 MFMA cost.
+    // The recomputed slice spectra on the FP32 MFMA, as the FP32 forward computes them
+    // (so every gain, and the side of the Wiener gain's kink a coefficient lies on, is the
+    // forward's); the linear adjoint contractions in split BF16 (hi + lo operands, three
+    // products, FP32 accumulation: about 16 significant bits per operand).
     constexpr int kBP = 1;
     constexpr int kWbuf = T * 256 + 256;
     extern __shared__ __attribute__((aligned(16))) float smem[]; // float4 accesses
@@ -473,9 +474,9 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
             for (int s = 0; s < 4; s++) {
                 wv[s] = win[t][s] * (raw4_value<float>(nxt[t], s) - boff);
             }
-            const Opnd in = make_opnd<kBP>(wv[0], wv[1], wv[2], wv[3]);
-            const floatx4 d1 = mm16<kBP>(in, WXo, zero);
-            d2[t] = mm16<kBP>(WYo, make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
+            // WXo and WYo carry their FP32 values beside the split halves
+            const floatx4 d1 = mm16<0>(make_opnd<0>(wv[0], wv[1], wv[2], wv[3]), WXo, zero);
+            d2[t] = mm16<0>(WYo, make_opnd<0>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
         {
             // centre-indexed temporal basis from the sums / differences of slices R + k
@@ -692,20 +693,39 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
 // TK.  The input-gradient accumulator of each virtual frame is a ring of ring_bands bands
 // of S rows; band r is final once block row r is done and is then added to the input
 // gradient with global atomics (mirror padding, temporal clamp) and cleared.
-// Split-BF16 basis operands of the cross-frame backward, kept in LDS: [b][lane] as the
-// packed hi and lo halves of a lane's Opnd.
-constexpr int kBaseWX = 0, kBaseWY = 1, kBaseWYT = 2, kBaseWXT = 3, kBases = 4;
+// Basis operands of the cross-frame backward, kept in LDS: [b][lane] as a lane's four
+// FP32 values (PREC 0) or the packed hi and lo halves of its split-BF16 Opnd (PREC 1).
+// WX, WY: the spatial forward in the recompute's precision; GX, GY: the same factors in
+// the output-gradient contraction's; WYT, WXT: the adjoint's.
+constexpr int kBaseWX = 0, kBaseWY = 1, kBaseWYT = 2, kBaseWXT = 3, kBaseGX = 4, kBaseGY = 5, kBases = 6;
 
+template <int PREC>
 __device__ __forceinline__ u32x4 opnd_bits(const Opnd& o) {
-    const u32x2 h = __builtin_bit_cast(u32x2, o.hi);
-    const u32x2 l = __builtin_bit_cast(u32x2, o.lo);
-    return u32x4{h.x, h.y, l.x, l.y};
+    if constexpr (PREC == 0) {
+        return u32x4{__builtin_bit_cast(unsigned, o.f[0]), __builtin_bit_cast(unsigned, o.f[1]),
+                     __builtin_bit_cast(unsigned, o.f[2]), __builtin_bit_cast(unsigned, o.f[3])};
+    } else {
+        const u32x2 h = __builtin_bit_cast(u32x2, o.hi);
+        const u32x2 l = __builtin_bit_cast(u32x2, o.lo);
+        return u32x4{h.x, h.y, l.x, l.y};
+    }
 }
 
+template <int PREC>
 __device__ __forceinline__ Opnd opnd_from_bits(u32x4 w) {
     Opnd o;
-    o.hi = __builtin_bit_cast(bf16x4, u32x2{w.x, w.y});
-    o.lo = __builtin_bit_cast(bf16x4, u32x2{w.z, w.w});
+    if constexpr (PREC == 0) {
+        // the whole vector cast at once: this compiler's __builtin_bit_cast of one element
+        // of an ext_vector (w.y, w[1]) reads element 0
+        const floatx4 f = __builtin_bit_cast(floatx4, w);
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            o.f[s] = f[s];
+        }
+    } else {
+        o.hi = __builtin_bit_cast(bf16x4, u32x2{w.x, w.y});
+        o.lo = __builtin_bit_cast(bf16x4, u32x2{w.z, w.w});
+    }
     return o;
 }
 
@@ -714,7 +734,14 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(R <= 1
 void backward_reuse_kernel(LaunchArgs a) {
     static_assert(R >= 1, "the cross-frame backward needs radius >= 1");
     constexpr int T = 2 * R + 1;
-    constexpr int kBP = 1; // split-BF16 contractions, as backward_kernel
+    // Contraction precision: the recomputed slice spectra on the FP32 MFMA, as the FP32
+    // forward computes them, so that every gain (and which side of the Wiener gain's kink
+    // at psd = sigma a coefficient lies on) is the forward's; the two linear adjoint
+    // contractions (output gradient -> dV, spectrum gradient -> input gradient) in split
+    // BF16.  Measured on 8 clips of 256x256x16: 1.07 ms per dispatch, input gradient
+    // within 5e-6 of FP64 (norm-wise); split BF16 throughout: 1.03 ms, 3.7e-4 (kink
+    // events); FP32 throughout: 1.14 ms, 1.1e-6.
+    constexpr int kFP = 0, kDP = 1, kAP = 1;
     constexpr int kWbuf = T * 256 + 256;
     extern __shared__ __attribute__((aligned(16))) float smem[]; // float4 accesses
 
@@ -790,18 +817,23 @@ void backward_reuse_kernel(LaunchArgs a) {
             wyT[s] = a.consts[kOffSWY + (4 * lg + s) * 16 + l16];
             wxT[s] = a.consts[kOffSWX + l16 * 16 + 4 * lg + s] * as_c;
         }
-        bases[kBaseWX * 64 + lane] = opnd_bits(make_opnd<kBP>(wx[0], wx[1], wx[2], wx[3]));
-        bases[kBaseWY * 64 + lane] = opnd_bits(make_opnd<kBP>(wy[0], wy[1], wy[2], wy[3]));
-        bases[kBaseWYT * 64 + lane] = opnd_bits(make_opnd<kBP>(wyT[0], wyT[1], wyT[2], wyT[3]));
-        bases[kBaseWXT * 64 + lane] = opnd_bits(make_opnd<kBP>(wxT[0], wxT[1], wxT[2], wxT[3]));
+        bases[kBaseWX * 64 + lane] = opnd_bits<kFP>(make_opnd<kFP>(wx[0], wx[1], wx[2], wx[3]));
+        bases[kBaseWY * 64 + lane] = opnd_bits<kFP>(make_opnd<kFP>(wy[0], wy[1], wy[2], wy[3]));
+        bases[kBaseGX * 64 + lane] = opnd_bits<kDP>(make_opnd<kDP>(wx[0], wx[1], wx[2], wx[3]));
+        bases[kBaseGY * 64 + lane] = opnd_bits<kDP>(make_opnd<kDP>(wy[0], wy[1], wy[2], wy[3]));
+        bases[kBaseWYT * 64 + lane] = opnd_bits<kAP>(make_opnd<kAP>(wyT[0], wyT[1], wyT[2], wyT[3]));
+        bases[kBaseWXT * 64 + lane] = opnd_bits<kAP>(make_opnd<kAP>(wxT[0], wxT[1], wxT[2], wxT[3]));
     }
     // The operand b of this lane, read from LDS at each use (the index is opaque to the
     // optimizer, which would otherwise hoist the reads out of the loops).
-    auto basis = [&](int b) {
+    auto basis = [&](auto prec, int b) {
         int i = b * 64 + lane;
         asm volatile("" : "+v"(i));
-        return opnd_from_bits(bases[i]);
+        return opnd_from_bits<decltype(prec)::value>(bases[i]);
     };
+    using FwdP = std::integral_constant<int, kFP>;
+    using DvP = std::integral_constant<int, kDP>;
+    using AdjP = std::integral_constant<int, kAP>;
     float wt[T][T];
 #pragma unroll
     for (int j = 0; j < T; j++) {
@@ -1088,8 +1120,8 @@ void backward_reuse_kernel(LaunchArgs a) {
             for (int s = 0; s < 4; s++) {
                 sv[s] = raw4_value<float>(cur[t], s) - boff;
             }
-            const floatx4 d1 = mm16<kBP>(make_opnd<kBP>(sv[0], sv[1], sv[2], sv[3]), basis(kBaseWX), zero);
-            d2[t] = mm16<kBP>(basis(kBaseWY), make_opnd<kBP>(d1[0], d1[1], d1[2], d1[3]), zero);
+            const floatx4 d1 = mm16<kFP>(make_opnd<kFP>(sv[0], sv[1], sv[2], sv[3]), basis(FwdP(), kBaseWX), zero);
+            d2[t] = mm16<kFP>(basis(FwdP(), kBaseWY), make_opnd<kFP>(d1[0], d1[1], d1[2], d1[3]), zero);
         }
         // ---- temporal real basis (centre-indexed, separable form), staged as [j][p][q][b*2+c] ----
         {
@@ -1135,8 +1167,8 @@ void backward_reuse_kernel(LaunchArgs a) {
                     dov[s] = ((cur_ok >> s) & 1u) ? dov[s] : 0.0f;
                 }
             }
-            const floatx4 pm = mm16<kBP>(make_opnd<kBP>(dov[0], dov[1], dov[2], dov[3]), basis(kBaseWX), zero);
-            const floatx4 dv = mm16<kBP>(basis(kBaseWY), make_opnd<kBP>(pm[0], pm[1], pm[2], pm[3]), zero);
+            const floatx4 pm = mm16<kDP>(make_opnd<kDP>(dov[0], dov[1], dov[2], dov[3]), basis(DvP(), kBaseGX), zero);
+            const floatx4 dv = mm16<kDP>(basis(DvP(), kBaseGY), make_opnd<kDP>(pm[0], pm[1], pm[2], pm[3]), zero);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int m = 4 * lg + i;
@@ -1176,8 +1208,8 @@ void backward_reuse_kernel(LaunchArgs a) {
             if (s > 0 && !last) {
                 continue;
             }
-            const floatx4 r1 = mm16<kBP>(make_opnd<kBP>(gs[s][0], gs[s][1], gs[s][2], gs[s][3]), basis(kBaseWYT), zero);
-            const floatx4 dbt = mm16<kBP>(basis(kBaseWXT), make_opnd<kBP>(r1[0], r1[1], r1[2], r1[3]), zero);
+            const floatx4 r1 = mm16<kAP>(make_opnd<kAP>(gs[s][0], gs[s][1], gs[s][2], gs[s][3]), basis(AdjP(), kBaseWYT), zero);
+            const floatx4 dbt = mm16<kAP>(basis(AdjP(), kBaseWXT), make_opnd<kAP>(r1[0], r1[1], r1[2], r1[3]), zero);
             // row l16, columns 4*(lane/16)..+3, at offset s*256 + wave*S of the wave's buffer
             float* srow = wbuf + s * 256 + wave * S + l16 * 16 + 4 * lg;
 #pragma unroll
