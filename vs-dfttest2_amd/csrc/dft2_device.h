@@ -91,19 +91,12 @@ __device__ __forceinline__ floatx4 mm16(const Opnd& A, const Opnd& B, floatx4 ac
     return acc;
 }
 
-// Split-BF16 contraction whose A operand is exact in BF16 (integers of at most 8
-// significant bits): A*B = A*Bhi + A*Blo, two BF16 MFMA.  Only A.hi is set.
+// An A operand that is exact in BF16 (integers of at most 8 significant bits): only
+// A.hi is set.
 __device__ __forceinline__ Opnd make_opnd_exact(float a0, float a1, float a2, float a3) {
     Opnd o;
     o.hi = __builtin_bit_cast(bf16x4, (static_cast<unsigned long long>(pack_bf16(a2, a3)) << 32) | pack_bf16(a0, a1));
     return o;
-}
-
-__device__ __forceinline__ floatx4 mm16_exact_a(const Opnd& A, const Opnd& B, floatx4 acc) {
-    const shortx4 ahi = __builtin_bit_cast(shortx4, A.hi);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.hi), acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, __builtin_bit_cast(shortx4, B.lo), acc, 0, 0, 0);
-    return acc;
 }
 
 // FP32 contraction whose A operand is exact in BF16, on the BF16 MFMA.  An FP32 value is
@@ -197,6 +190,75 @@ __device__ __forceinline__ floatx4 mm16_bytes(bf16x8 A, const OpndW& B, floatx4 
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B.mid, acc, 0, 0, 0);
     }
     acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B.hi, acc, 0, 0, 0);
+    return acc;
+}
+
+// Contraction whose A operand is made of integers of at most 8 bits, on the FP16 MFMA
+// (the split form's first contraction for 8-bit samples).
+// A byte b is exact in FP16, and so are the parts of an FP32 basis value split into FP16
+// (11 significant bits each; three parts hold the 24 bits of an FP32 value, two parts 22),
+// so every product is exact and the sums are FP32.  The basis is scaled by kBasisH16
+// (a power of two) before the split, which keeps its parts clear of FP16's subnormal
+// range for all but the smallest values (whose lost bits are far below the FP32 sums'
+// rounding); the caller takes the scale out of the next contraction's operand.
+typedef _Float16 halfx4 __attribute__((ext_vector_type(4)));
+typedef _Float16 halfx2 __attribute__((ext_vector_type(2)));
+constexpr float kBasisH16 = 4096.0f;
+
+struct OpndH {
+    halfx4 hi, mid, lo;
+};
+
+__device__ __forceinline__ halfx4 pack_h4(float a0, float a1, float a2, float a3) {
+    return halfx4{static_cast<_Float16>(a0), static_cast<_Float16>(a1), static_cast<_Float16>(a2),
+                  static_cast<_Float16>(a3)};
+}
+
+// PREC 0: three FP16 parts of kBasisH16 * a (smallest first in mm16_h); PREC 1: two.
+template <int PREC>
+__device__ __forceinline__ OpndH make_opndh(float a0, float a1, float a2, float a3) {
+    const float v[4] = {a0 * kBasisH16, a1 * kBasisH16, a2 * kBasisH16, a3 * kBasisH16};
+    float r[4], q[4];
+    OpndH o;
+    o.hi = pack_h4(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        r[i] = v[i] - static_cast<float>(o.hi[i]); // exact: the part left below hi's bits
+    }
+    if constexpr (PREC == 1) {
+        o.lo = pack_h4(r[0], r[1], r[2], r[3]);
+    } else {
+        o.mid = pack_h4(r[0], r[1], r[2], r[3]);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            q[i] = r[i] - static_cast<float>(o.mid[i]);
+        }
+        o.lo = pack_h4(q[0], q[1], q[2], q[3]);
+    }
+    return o;
+}
+
+// The four 8-bit samples of a dword as FP16: each byte placed under the exponent of 1024
+// (FP16 bits 0x6400 | b = 1024 + b, exact: 10 mantissa bits) by one byte permute per
+// pair, then 1024 taken off two at a time (exact).  Four instructions for four samples.
+__device__ __forceinline__ halfx4 bytes_h(unsigned d) {
+    const unsigned k = 0x64646464u;
+    // v_perm_b32 selectors: 0..3 pick bytes of the second operand (d), 4..7 of the first (k)
+    const unsigned p01 = __builtin_amdgcn_perm(k, d, 0x04010400u); // [b0, 0x64, b1, 0x64]
+    const unsigned p23 = __builtin_amdgcn_perm(k, d, 0x04030402u); // [b2, 0x64, b3, 0x64]
+    const halfx2 off = {static_cast<_Float16>(1024.0f), static_cast<_Float16>(1024.0f)};
+    const halfx2 h01 = __builtin_bit_cast(halfx2, p01) - off;
+    const halfx2 h23 = __builtin_bit_cast(halfx2, p23) - off;
+    return halfx4{h01[0], h01[1], h23[0], h23[1]};
+}
+
+template <int PREC>
+__device__ __forceinline__ floatx4 mm16_h(halfx4 A, const OpndH& B, floatx4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(A, B.lo, acc, 0, 0, 0);
+    if constexpr (PREC == 0) {
+        acc = __builtin_amdgcn_mfma_f32_16x16x16f16(A, B.mid, acc, 0, 0, 0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16f16(A, B.hi, acc, 0, 0, 0);
     return acc;
 }
 
