@@ -577,8 +577,9 @@ def bench_frame_host(args):
     DFTTestGetFrame runs it (hiprtc_source/source.cpp:445-692).  1080p YUV420 u8, 48 frames;
     one caller thread on one stream slot, and four threads on four slots (VapourSynth's
     fmParallel); pageable (numpy) and pinned (hipHostMalloc through torch) host memory; and
-    the same through dft2_process_frame_keyed, which uploads each source plane once.
-    PCIe-inclusive frames per second; never the headline value."""
+    the same through dft2_process_frame_keyed, which uploads each source plane once.  The
+    timed run goes four times through the clip's frames, as 192 output frames whose keys
+    are all distinct.  PCIe-inclusive frames per second; never the headline value."""
     import threading
 
     import numpy as np
@@ -588,6 +589,7 @@ def bench_frame_host(args):
     from dfttest2_amd.params import dfttest_params
 
     frames = 48
+    passes = 4
     p = dfttest_params()
     t_len, r = p.t_len, p.t_len // 2
     base = synth.yuv420_planes(frames, WIDTH, HEIGHT, seed=1, bits=8, sd_lsb=6.0)
@@ -603,13 +605,14 @@ def bench_frame_host(args):
             ptr = lambda a, n: a[n].ctypes.data  # noqa: E731
 
         def run(plan, frame_ids, keyed):
-            for n in frame_ids:
+            for m in frame_ids:  # output frame m: frame n of pass m // frames
+                n, base = m % frames, m - m % frames
                 for k, (pl, out) in enumerate(zip(planes, outs)):
                     _, h, w = pl.shape
                     idx = [min(max(n - r + t, 0), frames - 1) for t in range(t_len)]
                     srcs = [ptr(pl, i) for i in idx]
                     if keyed:  # dft2_process_frame_keyed: key = (frame, plane)
-                        plan.process_frame_keyed(srcs, [4 * i + k for i in idx], w, ptr(out, n), w, w, h)
+                        plan.process_frame_keyed(srcs, [4 * (base + i) + k for i in idx], w, ptr(out, n), w, w, h)
                     else:
                         plan.process_frame(srcs, w, ptr(out, n), w, w, h, memory=1)
 
@@ -620,10 +623,11 @@ def bench_frame_host(args):
                 if keyed:
                     plan.forget_frames()
                 t0 = time.perf_counter()
+                total = frames * passes
                 if threads == 1:
-                    run(plan, range(frames), keyed)
+                    run(plan, range(total), keyed)
                 else:
-                    pool = [threading.Thread(target=run, args=(plan, range(i, frames, threads), keyed))
+                    pool = [threading.Thread(target=run, args=(plan, range(i, total, threads), keyed))
                             for i in range(threads)]
                     for th in pool:
                         th.start()
@@ -632,10 +636,10 @@ def bench_frame_host(args):
                 dt = time.perf_counter() - t0
                 plan.close()
                 name = f"{memory}_{threads}thread" + ("_keyed" if keyed else "")
-                results[name] = {"fps": round(frames / dt, 1), "ms_per_frame": round(dt / frames * 1e3, 3)}
-                log(f"frame_host {memory}, {threads} thread(s){', keyed' if keyed else ''}: {frames / dt:.1f} fps")
+                results[name] = {"fps": round(total / dt, 1), "ms_per_frame": round(dt / total * 1e3, 3)}
+                log(f"frame_host {memory}, {threads} thread(s){', keyed' if keyed else ''}: {total / dt:.1f} fps")
     print(json.dumps({"metric": METRIC, "workload": "frame_host: dft2_process_frame (and _keyed) per frame and plane "
-                      "on host planes, 1080p YUV420 u8, 48 frames, DFTTest defaults", "unit": "fps",
+                      "on host planes, 1080p YUV420 u8, 4 passes over 48 frames, DFTTest defaults", "unit": "fps",
                       "results": results}), flush=True)
     return 0
 

@@ -1639,17 +1639,27 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
     // the least recently used plane no call uses, which this call then uploads.  A plane
     // another call is still uploading on another stream is not waited for: this call
     // uploads its own copy (waiting, on the host for the upload to be queued and on the
-    // GPU across streams, made concurrent calls slower than uploading again).  The same
+    // GPU across streams, made concurrent calls slower than uploading again; waiting on
+    // the GPU only, for uploads already queued, was no faster).  The same
     // key twice (the temporal clamp at a clip's ends) shares one plane.  The pool holds
     // 3 (T + 1) planes per stream slot, so that a caller going through consecutive frames
     // of three planes (YUV) finds the T - 1 planes it uploaded for each plane's previous
-    // frame; a call holds at most T, so a free plane always exists.
+    // frame; a call holds at most T, so a free plane always exists.  The pool's device
+    // memory and events are made by the first call, at its plane size (a larger plane
+    // later grows the plane it lands in), not by the calls that first use each plane.
     FramePlaneClaim claim{p};
     claim.n = T;
     {
         std::lock_guard<std::mutex> lk(p->mu);
         if (p->frame_planes.empty()) {
             p->frame_planes.resize(static_cast<size_t>(3 * (T + 1)) * p->slots.size());
+            for (auto& fp : p->frame_planes) {
+                rc = ensure_device(&fp.dev, &fp.cap, static_cast<size_t>(plane));
+                if (rc) {
+                    return rc;
+                }
+                HIP_TRY(hipEventCreateWithFlags(&fp.ready, hipEventDisableTiming));
+            }
         }
         for (int t = 0; t < T; t++) {
             for (int u = 0; u < t && !claim.use[t]; u++) {
