@@ -370,34 +370,6 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t frame_rsrc(const void* base, u
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes), kBufferDword3);
 }
 
-// Raw sample as loaded: the integer sample itself, or the bits of a float sample.
-template <typename Tin>
-using raw_t = typename std::conditional<sizeof(Tin) == 4, unsigned, Tin>::type;
-
-template <typename Tin>
-__device__ __forceinline__ raw_t<Tin> load_raw(__amdgpu_buffer_rsrc_t r, unsigned off_bytes);
-template <>
-__device__ __forceinline__ uint8_t load_raw<uint8_t>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
-    return __builtin_amdgcn_raw_buffer_load_b8(r, off_bytes, 0, 0);
-}
-template <>
-__device__ __forceinline__ uint16_t load_raw<uint16_t>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
-    return __builtin_amdgcn_raw_buffer_load_b16(r, off_bytes, 0, 0);
-}
-template <>
-__device__ __forceinline__ unsigned load_raw<float>(__amdgpu_buffer_rsrc_t r, unsigned off_bytes) {
-    return __builtin_amdgcn_raw_buffer_load_b32(r, off_bytes, 0, 0);
-}
-
-template <typename Tin>
-__device__ __forceinline__ float raw_value(raw_t<Tin> v) {
-    if constexpr (sizeof(Tin) == 4) {
-        return __builtin_bit_cast(float, v);
-    } else {
-        return static_cast<float>(v);
-    }
-}
-
 // Four consecutive samples (columns 4*(lane/16)..+3) of one lane and slice as raw bits:
 // one dword of 4 bytes, two dwords of 2 halves, or four float dwords.  They stay packed
 // until the samples are used, so a prefetch keeps its loads in flight and holds T
@@ -471,16 +443,6 @@ __device__ __forceinline__ float raw4_value(const raw4<Tin>& x, int s) {
     } else {
         return __builtin_bit_cast(float, x.v[s]);
     }
-}
-
-// Source planes live in device global memory; loading through an address-space-1
-// pointer gives global (not flat) loads.
-template <typename Tin>
-using gptr_t = const __attribute__((address_space(1))) Tin*;
-
-template <typename Tin>
-__device__ __forceinline__ gptr_t<Tin> to_global(const Tin* p) {
-    return (gptr_t<Tin>)(p);
 }
 
 // Gain of one coefficient (cpu_source/kernel.hpp:1826-1890).  FTM 0 and 1 are the
