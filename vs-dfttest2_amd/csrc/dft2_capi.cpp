@@ -1439,15 +1439,22 @@ int check_frame_call(const dft2_plan* p, const void* const* src_frames, long lon
     return check_mirror(height, p->block_size, p->block_step, "height");
 }
 
-// The source planes a dft2_process_frame_keyed call uses, released when the call ends
-// (its work is complete then: the call waits for its stream).  A plane this call was to
-// upload and did not is invalidated.
+// The source planes a dft2_process_frame_keyed call uses, released when the call ends.
+// Its work is complete then: a call that returns normally has waited for its stream, and
+// one that fails waits here, so that no upload or kernel of it still touches a plane
+// another call may claim next.  A plane this call was to upload and did not is
+// invalidated.
 struct FramePlaneClaim {
     dft2_plan* p;
+    hipStream_t st = nullptr;
+    bool synced = false;
     FramePlane* use[kMaxFramePtrs] = {};
     bool mine[kMaxFramePtrs] = {};
     int n = 0;
     ~FramePlaneClaim() {
+        if (!synced && st) {
+            (void)hipStreamSynchronize(st); // an error path: the call's own error stands
+        }
         {
             std::lock_guard<std::mutex> lk(p->mu);
             for (int t = 0; t < n; t++) {
@@ -1649,6 +1656,7 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
     // later grows the plane it lands in), not by the calls that first use each plane.
     FramePlaneClaim claim{p};
     claim.n = T;
+    claim.st = st;
     {
         std::lock_guard<std::mutex> lk(p->mu);
         if (p->frame_planes.empty()) {
@@ -1781,6 +1789,7 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
     }
     HIP_TRY(hipEventRecord(slot->done, st));
     HIP_TRY(hipStreamSynchronize(st));
+    claim.synced = true;
     if (!pinned_dst) {
         char* out = static_cast<char*>(dst);
         for (int y = 0; y < height; y++) {
