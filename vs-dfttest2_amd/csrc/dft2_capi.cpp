@@ -343,10 +343,21 @@ void release_slot(dft2_plan* p, Slot* s) {
     p->cv.notify_one();
 }
 
+// Holds a stream slot for one call.  Once the call may queue work on stream st it sets
+// st; a call that returns early after that (an error) leaves `recorded` false, and the
+// slot's done event is recorded here, so that the slot's next call, on whatever stream,
+// still waits for the work this one queued before reusing the slot's buffers.
 struct SlotGuard {
     dft2_plan* p;
     Slot* s;
-    ~SlotGuard() { release_slot(p, s); }
+    hipStream_t st = nullptr;
+    bool recorded = false;
+    ~SlotGuard() {
+        if (st && !recorded) {
+            (void)hipEventRecord(s->done, st);
+        }
+        release_slot(p, s);
+    }
 };
 
 int ensure_device(void** ptr, size_t* cap, size_t bytes) {
@@ -1183,11 +1194,13 @@ int dft2_process_batch_sigma(dft2_plan* p, const dft2_job* jobs, int n_jobs, con
     // The slot's buffers (scratch, job tables, gain table) may still be in use by its
     // previous call, queued on another stream: order this call after it.
     HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    guard.st = st;
     rc = run_jobs(p, slot, jd, st, sigma_dev, sigma_scale);
     if (rc) {
         return rc;
     }
     HIP_TRY(hipEventRecord(slot->done, st));
+    guard.recorded = true;
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));
     }
@@ -1234,11 +1247,13 @@ int dft2_process_batch_decisions(dft2_plan* p, const dft2_job* jobs, int n_jobs,
     SlotGuard guard{p, slot};
     hipStream_t st = call_stream(stream, slot->stream);
     HIP_TRY(hipStreamWaitEvent(st, slot->done, 0)); // the slot's previous call, on any stream
+    guard.st = st;
     rc = run_jobs(p, slot, jd, st, nullptr, 1.0f, decisions_dev);
     if (rc) {
         return rc;
     }
     HIP_TRY(hipEventRecord(slot->done, st));
+    guard.recorded = true;
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));
     }
@@ -1382,6 +1397,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     // The slot's buffers (scratch, job tables, gain table) may still be in use by its
     // previous call, queued on another stream: order this call after it.
     HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    guard.st = st;
     JobDev* d_table = nullptr;
     rc = upload_jobs(slot, jd.data(), sizeof(JobDev) * jd.size(), st, &d_table);
     if (rc) {
@@ -1405,6 +1421,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     }
     HIP_TRY(launch_backward(a, p->radius, items, lds, st));
     HIP_TRY(hipEventRecord(slot->done, st));
+    guard.recorded = true;
     if (!stream) {
         HIP_TRY(hipStreamSynchronize(st));
     }
@@ -1520,11 +1537,13 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
         SlotGuard guard{p, slot};
         hipStream_t st = call_stream(stream, slot->stream);
         HIP_TRY(hipStreamWaitEvent(st, slot->done, 0)); // the slot's previous call, on any stream
+        guard.st = st;
         rc = run_jobs(p, slot, {j}, st);
         if (rc) {
             return rc;
         }
         HIP_TRY(hipEventRecord(slot->done, st));
+        guard.recorded = true;
         if (!stream) {
             HIP_TRY(hipStreamSynchronize(st));
         }
@@ -1549,6 +1568,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
     // The slot's previous call may have run on a caller's stream: wait for it before the
     // staging buffers and scratch are reused.
     HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    guard.st = st;
     HIP_TRY(hipStreamSynchronize(st));
     rc = ensure_device(&slot->stage, &slot->stage_bytes, static_cast<size_t>(plane) * (T + 1));
     if (rc) {
@@ -1596,6 +1616,7 @@ int dft2_process_frame(dft2_plan* p, const void* const* src_frames, long long sr
                                hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(hipEventRecord(slot->done, st));
+    guard.recorded = true;
     HIP_TRY(hipStreamSynchronize(st));
     if (!pinned) {
         char* out = static_cast<char*>(dst);
@@ -1630,6 +1651,7 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
     // The slot's previous call may have run on a caller's stream: wait for it before the
     // staging buffers and scratch are reused.
     HIP_TRY(hipStreamWaitEvent(st, slot->done, 0));
+    guard.st = st;
     HIP_TRY(hipStreamSynchronize(st));
     rc = ensure_device(&slot->stage, &slot->stage_bytes, static_cast<size_t>(plane));
     if (rc) {
@@ -1788,6 +1810,7 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
         HIP_TRY(hipMemcpyAsync(h_stage + T * plane, slot->stage, static_cast<size_t>(plane), hipMemcpyDeviceToHost, st));
     }
     HIP_TRY(hipEventRecord(slot->done, st));
+    guard.recorded = true;
     HIP_TRY(hipStreamSynchronize(st));
     claim.synced = true;
     if (!pinned_dst) {
