@@ -98,11 +98,17 @@ TileShape make_tile(const dft2_plan* p, int bw, int bh) {
 }
 
 // Cross-frame form of plan p: frames per work item and tile.  It exists for separable
-// windows at radius >= 1.  Default: 8 frames per item from radius 2 (tbsize 5 and 7, where
-// the spatial transforms of T slices take most MFMA cycles; 4K 10-bit tbsize 5, FP32:
-// 23.2 -> 21.8 ms per 24 frames at 4, 20.4 at 8, the most rings for which two workgroups
-// still fit a CU); radius 1 keeps the one-frame form (its extra live registers cost a
-// third of the resident waves: 1080p 8-bit tbsize 3, 33.2 -> 41.3 ms at 4 frames).
+// windows at radius >= 1.  Default: 8 frames per item at radius 2 (tbsize 5, where the
+// spatial transforms of T slices take most MFMA cycles; whole clips in one launch, 1080p
+// and 4K YUV420 of 8-bit, 10-bit and float samples, both precisions: 8 frames 4-14 %
+// faster than the one-frame form, 2-10 % faster than 4 frames, the most rings for which
+// two workgroups still fit a CU) and at radius 3 for float samples (1080p 1,830 -> 2,259
+// fps FP32); radius 3 with integer samples keeps the one-frame form (its tbsize-7 spectra
+// take 153-166 VGPRs in the cross-frame form, one workgroup per CU: 1080p 8-bit 2,775 fps
+// one-frame against 2,452 at 8 frames, FP32; split BF16 3,229 against 2,631), and so does
+// radius 1 (its extra live registers cost a third of the resident waves: 1080p 8-bit
+// tbsize 3, 33.2 -> 41.3 ms at 4 frames).  Calls whose jobs have one output frame each
+// (per-frame calls) take the one-frame form whatever the plan's (run_jobs).
 // DFT2_REUSE=K and DFT2_REUSE_TILE=WxH override.  The tile: 8 blocks wide (one per wave
 // and pass), and the tallest candidate among those with the most resident workgroups:
 // its accumulators are rings of 16-row strips, so height costs no LDS, and taller tiles
@@ -112,7 +118,7 @@ void choose_reuse(dft2_plan* p) {
     if (!p->separable || p->radius < 1) {
         return;
     }
-    int k = p->radius >= 2 ? 8 : 1;
+    int k = p->radius == 2 || (p->radius >= 3 && p->sample_code == kF32) ? 8 : 1;
     if (const char* env = std::getenv("DFT2_REUSE")) {
         k = std::max(1, std::min(8, std::atoi(env)));
     }
@@ -502,6 +508,11 @@ int plan_chunks(const dft2_plan* p, const TileShape& ts, const std::vector<JobDe
     // A launch has fewer than 2^31 work items (32-bit item arithmetic in the kernels).
     frames_per_chunk = std::min<long long>(frames_per_chunk, (1LL << 31) / per_frame_items - 1);
     frames_per_chunk = std::max(1LL, std::min<long long>(frames_per_chunk, max_frames));
+    // Whole groups of reuse_k frames per chunk: a group cut by a chunk boundary repeats
+    // its first slices' transforms in the next chunk.
+    if (frames_per_chunk > reuse_k && frames_per_chunk < max_frames) {
+        frames_per_chunk -= frames_per_chunk % reuse_k;
+    }
     if (per_frame_items >= (1LL << 30)) {
         return fail(DFT2_ERR_ARG, "too many blocks in one frame of the batch");
     }
@@ -750,8 +761,14 @@ int run_jobs(dft2_plan* p, Slot* slot, const std::vector<JobDev>& jobs_in, hipSt
     if (p->generic) {
         return run_generic(p, slot, jobs_in, stream, decisions);
     }
-    // The cross-frame form when the plan takes it (it records no decisions).
-    const int reuse_k = decisions ? 1 : p->reuse_k;
+    // The cross-frame form when the plan takes it (it records no decisions), for calls
+    // with more than one output frame in some job: with one frame per job its items would
+    // transform every slice anyway and pay the rings and the form's registers for nothing.
+    int max_frames = 0;
+    for (const auto& j : jobs_in) {
+        max_frames = std::max(max_frames, j.f_count);
+    }
+    const int reuse_k = decisions || max_frames <= 1 ? 1 : p->reuse_k;
     const TileShape& ts = reuse_k > 1 ? p->reuse_tile : p->tile;
     // Job geometry counts tiles of p->tile; recount them for the form's tile.
     std::vector<JobDev> jobs_t;

@@ -161,7 +161,7 @@ struct dft2_plan {
     // frames per work item (1 = the one-frame form) and its tile.
     int reuse_k = 1;
     dft2::host::TileShape reuse_tile;
-    size_t scratch_cap = size_t(1) << 30;
+    size_t scratch_cap = size_t(4) << 30;
     float* d_consts = nullptr;
     std::vector<dft2::host::Slot> slots;
     std::mutex mu;
