@@ -1298,11 +1298,13 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
     // the per-tile work; tiles that leave one workgroup per CU are slower).  The
     // cross-frame form (separable window, radius >= 1) takes 8 output frames per work
     // item and keeps ring accumulators, whose LDS does not grow with the tile height; its
-    // tiles are the tallest of 16, 8 and 4 blocks that still give every resident
-    // workgroup slot at least 4 work items (8 clips of 256x256x16, tbsize 3: 1.30 ms in
-    // the one-frame form; 8 frames per item with 16-, 8- and 4-block tiles 1.30, 1.11 and
-    // 1.09 ms, 720, 1,296 and 2,448 items for 512 slots; 4 frames per item 1.20, 1.15 and
-    // 1.24 ms).  DFT2_BWD_REUSE=K and DFT2_BWD_TILE_H=H override (K = 1: one-frame form).
+    // tiles are the tallest of 16, 12, 8, 6, 5 and 4 blocks that still give every
+    // resident workgroup slot 3.5 work items (8 clips of 256x256x16, tbsize 3: 1.30 ms in
+    // the one-frame form; 8 frames per item with 16-, 8-, 6-, 5-, 4-, 3- and 2-block tiles
+    // 1.30, 1.11, 1.095, 1.060, 1.079, 1.122 and 1.227 ms, 720, 1,296, 1,728, 2,016,
+    // 2,448, 3,264 and 4,896 items for 512 slots; 4 frames per item 1.20, 1.15 and 1.24 ms
+    // at 16, 8 and 4 blocks).  DFT2_BWD_REUSE=K and DFT2_BWD_TILE_H=H override (K = 1:
+    // one-frame form).
     const int S = p->block_step;
     const int tbw = std::min(8, 84 / S + 1);
     auto region_w_of = [S](int tb) { return ((S * (tb - 1) + 16) + 3) / 4 * 4; };
@@ -1329,7 +1331,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
             return fail(DFT2_ERR_HIP, "the cross-frame backward kernel does not fit a CU");
         }
         const long long slots = static_cast<long long>(n_cu) * wg;
-        for (int h : {16, 8, 4}) {
+        for (int h : {16, 12, 8, 6, 5, 4}) {
             tbh = h;
             if (forced_h > 0) {
                 tbh = forced_h;
@@ -1341,7 +1343,7 @@ int dft2_backward_batch(dft2_plan* p, const dft2_grad_job* jobs, int n_jobs, con
                 const long long frames = std::max(0, jobs[i].frame_end - jobs[i].frame_begin);
                 n_items += (frames + reuse_k - 1) / reuse_k * ((nbx + tbw - 1) / tbw) * ((nby + h - 1) / h);
             }
-            if (n_items >= 4 * slots) {
+            if (2 * n_items >= 7 * slots) {
                 break;
             }
         }
