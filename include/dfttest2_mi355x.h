@@ -181,7 +181,13 @@ DFT2_API int dft2_process_frame_keyed(dft2_plan* plan, const void* const* src_fr
 DFT2_API int dft2_plan_forget_frames(dft2_plan* plan);
 
 /* Filters every job (device memory) with one fused launch per frame chunk, queued on
- * `stream` (NULL = the plan's own stream, and the call waits for completion). */
+ * `stream` (NULL = the plan's own stream, and the call waits for completion).  A chunk
+ * holds as many frames as the stream slot's tile scratch allows (environment variable
+ * DFT2_SCRATCH_MB at plan creation, default 4096), in whole groups of the cross-frame
+ * form's frames per work item.  Plans with a separable window take that form (8 output
+ * frames per work item, each source slice transformed once per item) at tbsize 5, and at
+ * tbsize 7 for float samples; DFT2_REUSE=K at plan creation overrides K (1 = one frame
+ * per item).  Calls whose jobs have one output frame each take the one-frame form. */
 DFT2_API int dft2_process_batch(dft2_plan* plan, const dft2_job* jobs, int n_jobs, void* stream);
 
 /* As dft2_process_batch, for callers that cannot allocate (code under stream capture,
