@@ -120,8 +120,13 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
     }
     JobDev jd[kMaxNoAllocJobs];
     // The plan's form, as dft2_process_batch takes it: one frame per work item, or the
-    // cross-frame form (reuse_k frames per item, its own tile).
-    const int reuse_k = p->reuse_k;
+    // cross-frame form (reuse_k frames per item, its own tile) for calls with more than
+    // one output frame in some job.
+    int call_frames = 0;
+    for (int i = 0; i < n_jobs; i++) {
+        call_frames = std::max(call_frames, jobs[i].frame_end - jobs[i].frame_begin);
+    }
+    const int reuse_k = call_frames <= 1 ? 1 : p->reuse_k;
     const TileShape& ts = reuse_k > 1 ? p->reuse_tile : p->tile;
     long long per_frame_items = 0;
     int max_frames = 0;
@@ -144,7 +149,11 @@ int dft2_process_batch_noalloc(dft2_plan* p, const dft2_job* jobs, int n_jobs, v
                     "one frame of these jobs needs %.1f MiB of tile scratch; the allocation-free entry holds %.1f MiB",
                     frame_bytes / 1048576.0, region_cap / 1048576.0);
     }
-    const int frames_per_chunk = static_cast<int>(std::min<size_t>(region_cap / frame_bytes, max_frames));
+    int frames_per_chunk = static_cast<int>(std::min<size_t>(region_cap / frame_bytes, max_frames));
+    // whole groups of reuse_k frames per chunk where a chunk holds more than one group
+    if (frames_per_chunk > reuse_k && frames_per_chunk < max_frames) {
+        frames_per_chunk -= frames_per_chunk % reuse_k;
+    }
     int rc = select_device(p);
     if (rc) {
         return rc;
