@@ -45,19 +45,31 @@ __device__ __forceinline__ float load_sample(const Tin* p) {
 }
 
 // One wave per block; waves of a workgroup work on consecutive blocks, each in its own LDS
-// buffers (two complex T x B x (B/2+1) arrays).  No workgroup barrier: a wave's LDS
-// accesses are ordered by wave barriers only.
+// buffers (two complex T x B x (B/2+1) arrays).  The twiddle tables (cos and sin of
+// 2 pi m / B and of 2 pi m / T) are copied to LDS behind those buffers once per workgroup:
+// read from global memory inside the DFT loops, with a lane-dependent index, every
+// multiply-add waited for a load.  After that copy's barrier there is no workgroup
+// barrier: a wave's LDS accesses are ordered by wave barriers only.
 template <typename Tin>
 __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
     extern __shared__ __attribute__((aligned(16))) float smem[];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int B = g.block_size, H = g.half, T = g.t_len, R = g.radius, S = g.block_step;
+    const int tbh = T * B * H;
+    float* tw = smem + static_cast<size_t>(blockDim.x >> 6) * 4 * tbh; // [cos_b B | sin_b B | cos_t T | sin_t T]
+    for (int i = threadIdx.x; i < 2 * (B + T); i += blockDim.x) {
+        tw[i] = i < B ? g.cos_b[i] : i < 2 * B ? g.sin_b[i - B] : i < 2 * B + T ? g.cos_t[i - 2 * B] : g.sin_t[i - 2 * B - T];
+    }
+    __syncthreads();
+    const float* cos_b = tw;
+    const float* sin_b = tw + B;
+    const float* cos_t = tw + 2 * B;
+    const float* sin_t = cos_t + T;
     const int item = blockIdx.x * (blockDim.x >> 6) + wave;
     if (item >= g.n_items) {
         return;
     }
-    const int B = g.block_size, H = g.half, T = g.t_len, R = g.radius, S = g.block_step;
-    const int tbh = T * B * H;
     float* x0 = smem + static_cast<size_t>(wave) * 4 * tbh; // complex [tbh] (first: real [T][B][B])
     float* x1 = x0 + 2 * tbh;                               // complex [tbh]
 
@@ -92,8 +104,8 @@ __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
         const float* row = x0 + ty * B;
         float re = 0.0f, im = 0.0f;
         for (int x = 0, m = 0; x < B; x++, m = (m + k >= B) ? m + k - B : m + k) {
-            re = fmaf(row[x], g.cos_b[m], re);
-            im = fmaf(-row[x], g.sin_b[m], im);
+            re = fmaf(row[x], cos_b[m], re);
+            im = fmaf(-row[x], sin_b[m], im);
         }
         x1[2 * e] = re;
         x1[2 * e + 1] = im;
@@ -108,7 +120,7 @@ __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
         const float* col = x1 + 2 * (t * B * H + k);
         float re = 0.0f, im = 0.0f;
         for (int y = 0, m = 0; y < B; y++, m = (m + ky >= B) ? m + ky - B : m + ky) {
-            const float a = col[2 * y * H], b = col[2 * y * H + 1], c = g.cos_b[m], s = g.sin_b[m];
+            const float a = col[2 * y * H], b = col[2 * y * H + 1], c = cos_b[m], s = sin_b[m];
             re = fmaf(a, c, fmaf(b, s, re));   // (a + ib)(c - is)
             im = fmaf(b, c, fmaf(-a, s, im));
         }
@@ -123,7 +135,7 @@ __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
         float re = 0.0f, im = 0.0f;
         for (int t = 0, m = 0; t < T; t++, m = (m + kt >= T) ? m + kt - T : m + kt) {
             const float a = x0[2 * (t * B * H + r2)], b = x0[2 * (t * B * H + r2) + 1];
-            const float c = g.cos_t[m], s = g.sin_t[m];
+            const float c = cos_t[m], s = sin_t[m];
             re = fmaf(a, c, fmaf(b, s, re));
             im = fmaf(b, c, fmaf(-a, s, im));
         }
@@ -159,7 +171,7 @@ __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
         float re = 0.0f, im = 0.0f;
         for (int kt = 0, m = 0; kt < T; kt++, m = (m + R >= T) ? m + R - T : m + R) {
             const float a = x1[2 * (kt * B * H + e)], b = x1[2 * (kt * B * H + e) + 1];
-            const float c = g.cos_t[m], s = g.sin_t[m];
+            const float c = cos_t[m], s = sin_t[m];
             re = fmaf(a, c, fmaf(-b, s, re));  // (a + ib)(c + is)
             im = fmaf(a, s, fmaf(b, c, im));
         }
@@ -174,7 +186,7 @@ __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
         float re = 0.0f, im = 0.0f;
         for (int ky = 0, m = 0; ky < B; ky++, m = (m + y >= B) ? m + y - B : m + y) {
             const float a = x0[2 * (ky * H + k)], b = x0[2 * (ky * H + k) + 1];
-            const float c = g.cos_b[m], s = g.sin_b[m];
+            const float c = cos_b[m], s = sin_b[m];
             re = fmaf(a, c, fmaf(-b, s, re));
             im = fmaf(a, s, fmaf(b, c, im));
         }
@@ -191,9 +203,9 @@ __global__ __launch_bounds__(256) void generic_block_kernel(GenericArgs g) {
         float sum = row[0];
         for (int k = 1, m = x; k < H; k++, m = (m + x >= B) ? m + x - B : m + x) {
             if (2 * k == B) {
-                sum = fmaf(row[2 * k], g.cos_b[m], sum); // Nyquist column: real part only
+                sum = fmaf(row[2 * k], cos_b[m], sum); // Nyquist column: real part only
             } else {
-                sum = fmaf(2.0f * row[2 * k], g.cos_b[m], fmaf(-2.0f * row[2 * k + 1], g.sin_b[m], sum));
+                sum = fmaf(2.0f * row[2 * k], cos_b[m], fmaf(-2.0f * row[2 * k + 1], sin_b[m], sum));
             }
         }
         out[e] = sum * g.syn[e];
@@ -242,7 +254,8 @@ __global__ __launch_bounds__(256) void generic_gather_kernel(GenericArgs g) {
 } // namespace
 
 hipError_t launch_generic_blocks(const GenericArgs& g, int sample_code, int waves_per_group, hipStream_t stream) {
-    const size_t lds = generic_lds_per_wave(g.block_size, g.t_len) * waves_per_group;
+    const size_t lds = generic_lds_per_wave(g.block_size, g.t_len) * waves_per_group +
+                       sizeof(float) * 2 * static_cast<size_t>(g.block_size + g.t_len);
     const dim3 grid(static_cast<unsigned>((g.n_items + waves_per_group - 1) / waves_per_group));
     const dim3 block(64 * waves_per_group);
     const void* kern = sample_code == kU8    ? reinterpret_cast<const void*>(generic_block_kernel<uint8_t>)
