@@ -2,7 +2,8 @@
 whole-clip path for sequential, shuffled and repeated frame orders, source planes of two
 geometries sharing the plan's pool, replacement of the least recently used planes,
 dft2_plan_forget_frames, pinned planes, concurrent calls from several threads over
-several stream slots, and the generic block path at radius 4 (also through
+several stream slots, padded rows (pageable and pinned, 8- and 10-bit), and the generic
+block path at radius 4 (also through
 dft2_process_frame on device planes, whose frame-pointer table covers 9 slices)."""
 
 import sys
@@ -92,6 +93,42 @@ def test_keyed_frames_from_pinned_planes():
         plan.process_frame_keyed([pinned[i].data_ptr() for i in idx], [4 * i for i in idx], 2 * w, out.data_ptr(),
                                  2 * w, w, h)
         np.testing.assert_array_equal(out.numpy().view(np.uint16), ref[n], err_msg=f"frame {n}")
+
+
+@pytest.mark.parametrize("memory", ["pageable", "pinned"])
+def test_keyed_frames_with_row_pitch(memory):
+    # Planes whose rows are padded (pitch 96 samples for 70-sample rows, as VapourSynth's
+    # frames are), source and output alike, 8-bit and 16-bit containers.
+    announce("keyed frames: padded rows", memory)
+    for bits in (8, 10):
+        p = dfttest_params()
+        x = synth.int_plane(5, 52, 70, seed=88 + bits, bits=bits, sd_lsb=6.0).numpy()
+        ref = filter_clip(VideoClip([x], "int", bits), p).planes[0]
+        plan = get_plan(p, "int", bits, Backend.MI355X(num_streams=1))
+        plan.forget_frames()
+        n_f, h, w = x.shape
+        pitch = 96
+        dt = x.dtype
+        if memory == "pinned":
+            tdt = torch.uint8 if dt == np.uint8 else torch.int16
+            bufs = [torch.zeros((h, pitch), dtype=tdt).pin_memory() for _ in range(n_f)]
+            views = [b.numpy().view(dt) for b in bufs]
+            out_t = torch.zeros((h, pitch), dtype=tdt).pin_memory()
+            out = out_t.numpy().view(dt)
+        else:
+            views = [np.zeros((h, pitch), dt) for _ in range(n_f)]
+            out = np.zeros((h, pitch), dt)
+        for i in range(n_f):
+            views[i][:, :w] = x[i]
+            views[i][:, w:] = dt.type(7)  # padding the filter must not read
+        bps = dt.itemsize
+        for n in range(n_f):
+            idx = [min(max(n - 1 + t, 0), n_f - 1) for t in range(3)]
+            out[:] = 0
+            plan.process_frame_keyed([views[i].ctypes.data for i in idx], [4 * i for i in idx], pitch * bps,
+                                     out.ctypes.data, pitch * bps, w, h)
+            np.testing.assert_array_equal(out[:, :w], ref[n], err_msg=f"{memory} {bits}-bit frame {n}")
+            assert not out[:, w:].any(), "samples written past the row"
 
 
 def test_keyed_frames_from_threads():
