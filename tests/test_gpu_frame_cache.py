@@ -3,8 +3,8 @@ whole-clip path for sequential, shuffled and repeated frame orders, source plane
 geometries sharing the plan's pool, replacement of the least recently used planes,
 dft2_plan_forget_frames, pinned planes, concurrent calls from several threads over
 several stream slots, padded rows (pageable and pinned, 8- and 10-bit), and the generic
-block path at radius 4 (also through
-dft2_process_frame on device planes, whose frame-pointer table covers 9 slices)."""
+block path at radius 4 (also through dft2_process_frame on device planes, whose
+frame-pointer table covers 9 slices)."""
 
 import sys
 import threading
