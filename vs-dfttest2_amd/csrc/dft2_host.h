@@ -76,6 +76,10 @@ struct DeviceTable {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    // dft2_process_frame_keyed: recorded after a call's source-plane uploads; the planes
+    // that call uploaded point at it (a later call's record only delays when they count
+    // as complete, since the slot's stream runs its calls in order)
+    hipEvent_t up_ready = nullptr;
     float* scratch = nullptr;
     size_t scratch_bytes = 0;
     // Recently uploaded job tables: a call whose table matches one of them byte for
@@ -107,7 +111,7 @@ struct FramePlane {
     unsigned long long last_use = 0;
     void* dev = nullptr;     // width * height * bps bytes, rows packed
     size_t cap = 0;
-    hipEvent_t ready = nullptr; // recorded after the upload, on the uploading call's stream
+    hipEvent_t ready = nullptr; // the uploading slot's up_ready (not owned), recorded after the upload
     hipStream_t up_stream = nullptr; // that stream: calls on it are ordered after the upload
 };
 
