@@ -22,6 +22,10 @@ The same JSON line also carries (default --workload all):
   parity            the GPU outputs against the reference's outputs on that sample (full
                     HD, all planes), and both 4K forms against the FP64 oracle on a check
                     clip, hard decisions replayed; exit status 1 if a bar is missed.
+
+Other workloads run alone, never in the default line: frame_host (per-frame calls on host
+planes, as a VapourSynth GetFrame makes them, PCIe included; plain and keyed entries,
+pageable and pinned planes, one and four caller threads).
 """
 
 from __future__ import annotations
