@@ -58,3 +58,20 @@ def test_frontend_diff_applies_to_the_reference(tmp_path):
     r = subprocess.run(["patch", "-p1", "-i", DIFF], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "class MI355X" in (tmp_path / "python" / "backends.py").read_text()
+
+
+def test_getframe_holds_the_largest_radius():
+    # GetFrame gathers T = 2 * radius + 1 source frames into stack arrays; dft2_plan_create
+    # accepts radius up to the generic path's maximum, so the arrays cover that T.
+    text = open(SRC).read()
+    internal = open(os.path.join(ROOT, "vs-dfttest2_amd", "csrc", "dft2_internal.h")).read()
+    host = open(os.path.join(ROOT, "vs-dfttest2_amd", "csrc", "dft2_host.h")).read()
+    m = re.search(r"kMaxGenericRadius\s*=\s*(\d+)", internal + host)
+    assert m
+    max_t = 2 * int(m.group(1)) + 1
+    slices = int(re.search(r"constexpr int kMaxSlices\s*=\s*(\d+)", text).group(1))
+    assert slices >= max_t
+    body = text[text.index("VS_CC GetFrame"):text.index("VS_CC Free")]
+    for decl in ("src[kMaxSlices]", "src_n[kMaxSlices]", "planes[kMaxSlices]", "keys[kMaxSlices]"):
+        assert decl in body, decl
+    assert not re.search(r"\w+\[\d+\]\s*=\s*\{\}", body)  # no other fixed-size per-slice arrays

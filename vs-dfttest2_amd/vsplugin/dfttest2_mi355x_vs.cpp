@@ -31,6 +31,10 @@
 
 namespace {
 
+// Source frames of one output frame at the largest radius dft2_plan_create accepts (7: the
+// generic block path), T = 2 * 7 + 1.
+constexpr int kMaxSlices = 15;
+
 struct DFTTestData {
     VSNode* node = nullptr;
     int radius = 0;
@@ -51,9 +55,9 @@ const VSFrame* VS_CC GetFrame(int n, int activation_reason, void* instance_data,
     if (activation_reason != arAllFramesReady) {
         return nullptr;
     }
-    const int t_len = 2 * d->radius + 1;
-    const VSFrame* src[7] = {};
-    int src_n[7] = {};
+    const int t_len = 2 * d->radius + 1; // <= kMaxSlices (Create keeps radius <= 7)
+    const VSFrame* src[kMaxSlices] = {};
+    int src_n[kMaxSlices] = {};
     for (int t = 0; t < t_len; t++) {
         src_n[t] = std::clamp(n - d->radius + t, 0, vi->numFrames - 1);
         src[t] = vsapi->getFrameFilter(src_n[t], d->node, frame_ctx);
@@ -68,8 +72,8 @@ const VSFrame* VS_CC GetFrame(int n, int activation_reason, void* instance_data,
         if (!d->process[plane]) {
             continue;
         }
-        const void* planes[7] = {};
-        unsigned long long keys[7] = {};
+        const void* planes[kMaxSlices] = {};
+        unsigned long long keys[kMaxSlices] = {};
         for (int t = 0; t < t_len; t++) {
             planes[t] = vsapi->getReadPtr(src[t], plane);
             keys[t] = static_cast<unsigned long long>(src_n[t]) * 4 + static_cast<unsigned long long>(plane);
@@ -171,6 +175,10 @@ void VS_CC Create(const VSMap* in, VSMap* out, void*, VSCore* core, const VSAPI*
     if (dft2_plan_create(&p, sample_type, vi->format.bitsPerSample, device_id, num_streams, precision, &d->plan) !=
         DFT2_OK) {
         return fail(dft2_last_error());
+    }
+    if (2 * p.radius + 1 > kMaxSlices) { // dft2_plan_create rejects radius > 7 already
+        dft2_plan_destroy(d->plan);
+        return fail("radius must be at most 7");
     }
     d->radius = p.radius;
     VSFilterDependency deps[] = {{d->node, rpGeneral}};
