@@ -5,6 +5,7 @@ against (a) the golden vectors made by the reference's own fused(), (b) the FP64
 oracle on the same seeded inputs, and (c) size-independent properties at full HD.
 """
 
+import dataclasses
 import os
 import sys
 
@@ -86,6 +87,23 @@ def test_random_params_vs_oracle(fmt, shape, kw):
     x, bits, is_float = make_plane(fmt, shape, seed=100 + len(kw))
     out = gpu_filter(x, p, bits, is_float)
     assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= tol(is_float)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split_bf16"])
+@pytest.mark.parametrize("step", [4, 5, 6])
+@pytest.mark.parametrize("fmt,shape", [("u8", (3, 52, 180)), ("f32", (3, 46, 44))])
+def test_block_steps_4_to_6(fmt, shape, step, precision):
+    # Block steps 4..6 take the one-frame kernel's per-lane gather (one pass column per
+    # lane, at most four covering blocks); 5 and 6 are reachable through the C ABI only
+    # (the front end rejects more than 50 % overlap with a step that does not divide 16).
+    # 180 columns make tiles whose last pass has fewer than eight blocks, 44 a tile
+    # narrower than one pass.
+    announce("block step", step, fmt, shape, precision)
+    p = dataclasses.replace(dfttest_params(), block_step=step)
+    x, bits, is_float = make_plane(fmt, shape, seed=300 + step)
+    out = gpu_filter(x, p, bits, is_float, Backend.MI355X(precision=precision))
+    bar = tol(is_float) * (5 if precision == "split_bf16" and is_float else 1)
+    assert max_err(out, onp.process_plane(x, p, bits, is_float)) <= bar
 
 
 @pytest.mark.parametrize("precision", ["fp32", "split_bf16"])
