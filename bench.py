@@ -52,7 +52,7 @@ SIMDS = 256 * 4                 # 256 CUs x 4 SIMDs
 # being what the same counters read for a plain copy kernel of known size
 # (profiles/fused_kernel_1080p_yuv420_u8_fp32.md).  A fixed figure of this build's kernel,
 # not measured inside the timed run.
-FUSED_TRAFFIC_BYTES = (1_018_604 * 2 + 4_797_214) * 1024
+FUSED_TRAFFIC_BYTES = (1_019_360 * 2 + 4_797_214) * 1024
 # MFMA pipe cycles per instruction on gfx950 (wave64): the FP32 16x16x4 and the BF16
 # 16x16x16 measured, the BF16 16x16x32 at the published dense BF16 rate.
 MFMA_CYCLES = {"f32_16x16x4": 32, "bf16_16x16x16": 16, "bf16_16x16x32": 16}
