@@ -613,13 +613,17 @@ void fused_kernel(LaunchArgs a) {
                 for (int i = 0; i < kRows; i++) {
                     const int dy = i * kWaves + wave;
                     old[i] = acc[(r * S + dy) * rw + pass * kWaves * S + lane];
-                    // blocks k0 .. k0 + 3 in order, as gather_head sums them
+                    // blocks k0 .. k0 + 3 in order, as gather_head sums them; all four read
+                    // (every address lies in the staging area), so that the reads pair up
+                    // and are not made conditional
                     const float* p = stage + dy * kOut + g_off;
+                    float v0 = p[0], v1 = p[kWbuf], v2 = p[2 * kWbuf], v3 = p[3 * kWbuf];
+                    asm volatile("" : "+v"(v0), "+v"(v1), "+v"(v2), "+v"(v3));
                     float s = 0.0f;
-                    s += gm0 ? p[0] : 0.0f;
-                    s += gm1 ? p[kWbuf] : 0.0f;
-                    s += gm2 ? p[2 * kWbuf] : 0.0f;
-                    s += gm3 ? p[3 * kWbuf] : 0.0f;
+                    s += gm0 ? v0 : 0.0f;
+                    s += gm1 ? v1 : 0.0f;
+                    s += gm2 ? v2 : 0.0f;
+                    s += gm3 ? v3 : 0.0f;
                     sum[i] = s;
                 }
 #pragma unroll
