@@ -200,10 +200,11 @@ void fused_kernel(LaunchArgs a) {
     // cover it, and where block k0's value sits, depend on the lane and S only, so they are
     // found once here (the masks as lane masks, no per-pass compares or divisions).  The
     // bounds are those of a full pass (kWaves blocks); a pass with fewer blocks reads zeros
-    // from the staging buffers of its idle waves, which clear them.  1080p u8, one launch
-    // per 240 frames: fused 33.3 -> 32.75 ms (FP32), 26.1 -> 25.4 ms (split BF16); the
-    // cross-frame form keeps the general gather (4K 10-bit tbsize 5, FP32: 94.0 -> 97.2 ms
-    // with this path).
+    // from the staging buffers of its idle waves, which clear them.  1080p u8 tbsize 3, per
+    // half-clip launch on one box: fused 16.71 -> 16.22 ms (FP32), 13.53 -> 13.21 ms (split
+    // BF16); tbsize 1 and 7 unchanged.  The cross-frame form keeps the general gather: its
+    // radius-2 instantiations spill more with this path (4K 10-bit tbsize 5, FP32: 94.0 ->
+    // 97.2 ms).
     const bool gfast = !REUSE && S >= 4 && 7 * S + 16 <= 64;
     const int g_lo = div_small(max(lane + S - 16, 0), a.step_magic);
     const int g_hi = min(div_small(lane, a.step_magic), kWaves - 1);
