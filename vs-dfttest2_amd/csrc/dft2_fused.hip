@@ -202,10 +202,15 @@ void fused_kernel(LaunchArgs a) {
     // bounds are those of a full pass (kWaves blocks); a pass with fewer blocks reads zeros
     // from the staging buffers of its idle waves, which clear them.  1080p u8 tbsize 3, per
     // half-clip launch on one box: fused 16.71 -> 16.22 ms (FP32), 13.53 -> 13.21 ms (split
-    // BF16); tbsize 1 and 7 unchanged.  The cross-frame form keeps the general gather: its
-    // radius-2 instantiations spill more with this path (4K 10-bit tbsize 5, FP32: 94.0 ->
-    // 97.2 ms).
-    const bool gfast = !REUSE && S >= 4 && 7 * S + 16 <= 64;
+    // BF16); tbsize 1 and 7 unchanged.  The path holds a VGPR and four lane masks for the
+    // kernel's life, so it is compiled only where that costs no spills and no waves: the
+    // one-item, separable, symmetric 8-bit instantiations of radius 0, 1 and 3.  The others
+    // keep the general gather: with this path the float ones at radius 1 spill (the
+    // differentiable forward, 8 x 256x256x16, 0.64 -> 0.69 ms), 16-bit, asymmetric,
+    // persistent and radius-2 ones lose a wave per SIMD, and the cross-frame form's radius-2
+    // ones spill more (4K 10-bit tbsize 5, FP32: 94.0 -> 97.2 ms).
+    constexpr bool kGatherLane = !REUSE && !PERSIST && !REC && SEP && sizeof(Tin) == 1 && ASYM == 0 && R != 2;
+    const bool gfast = kGatherLane && S >= 4 && 7 * S + 16 <= 64;
     const int g_lo = div_small(max(lane + S - 16, 0), a.step_magic);
     const int g_hi = min(div_small(lane, a.step_magic), kWaves - 1);
     const int g_k0 = max(g_hi - 3, 0);
