@@ -190,6 +190,27 @@ def test_frame_entry_host_and_device_match_batch():
         np.testing.assert_array_equal(dout.cpu().numpy(), full[fr])
 
 
+@pytest.mark.parametrize("step", [4, 5, 6])
+@pytest.mark.parametrize("tbsize", [1, 3, 7])
+def test_frame_entry_block_steps_match_batch(tbsize, step):
+    # Per-frame calls (one output frame per launch, host planes as GetFrame passes them)
+    # at the block steps whose overlap-add takes the per-lane gather in the 8-bit
+    # instantiations: the same bits as the whole-clip batch, within 1 LSB of the oracle.
+    announce("process_frame block step", step, "tbsize", tbsize)
+    p = dataclasses.replace(dfttest_params(tbsize=tbsize), block_step=step)
+    x, bits, _ = make_plane("u8", (5, 50, 140), 60 + step)
+    full = gpu_filter(x, p, bits, False)
+    assert max_err(full, onp.process_plane(x, p, bits, False)) <= 1
+    plan = get_plan(p, "int", 8, Backend.MI355X())
+    n, h, w = x.shape
+    r = p.radius
+    for fr in (0, n // 2, n - 1):
+        srcs = [np.ascontiguousarray(x[min(max(fr - r + t, 0), n - 1)]) for t in range(p.t_len)]
+        out = np.zeros((h, w), np.uint8)
+        plan.process_frame([s.ctypes.data for s in srcs], w, out.ctypes.data, w, w, h, memory=1)
+        np.testing.assert_array_equal(out, full[fr])
+
+
 @pytest.mark.parametrize("fmt", ["u8", "u10", "f32"])
 def test_frame_entry_pinned_host_planes(fmt):
     # Pinned host planes take the direct copies, pageable ones the pinned staging buffer:
